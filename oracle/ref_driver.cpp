@@ -1,0 +1,43 @@
+// Test-only driver around the reference's own sources (compiled from
+// /root/reference by oracle/build_ref.sh into oracle/_ref/).  It exercises
+//   Boundary::ApplyBc (Boundary.cpp:127-148, D3Q19 Zou-He inlet)
+// The reference's voxelizer (ComplexGeometry.cpp) is not built: it includes
+// mpi_call.h, whose member template at :143 redeclares the class template's
+// parameter T, which gcc and clang reject (icc accepted it).
+// Usage:
+//   ref_driver zouhe <in.bin> <out.bin>
+//       reads n then n*19 doubles, applies Boundary::ApplyBc to each node
+#include <cstdio>
+#include <cstdint>
+#include <fstream>
+#include <memory>
+#include <vector>
+#include <string>
+#include "LatticeModel.h"
+#include "Boundary.h"
+
+static int zouhe(const char* in, const char* out) {
+    std::ifstream i(in, std::ios::binary);
+    int64_t n = 0;
+    i.read((char*)&n, sizeof n);
+    std::vector<double> f((size_t)n * 19);
+    i.read((char*)f.data(), f.size() * sizeof(double));
+    auto velSet = std::make_shared<LatticeModel>();
+    for (int64_t m = 0; m < n; ++m) {
+        Boundary b(0, 0, 0, velSet, NodeType::Boundary);
+        b.BoundaryType_ = BoundaryType::Inlet;
+        b.m_distributions.assign(f.begin() + m * 19, f.begin() + (m + 1) * 19);
+        b.ApplyBc();
+        std::copy(b.m_distributions.begin(), b.m_distributions.end(), f.begin() + m * 19);
+    }
+    std::ofstream o(out, std::ios::binary);
+    o.write((const char*)f.data(), f.size() * sizeof(double));
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    int rc = 2;
+    if (argc == 4 && std::string(argv[1]) == "zouhe") rc = zouhe(argv[2], argv[3]);
+    else fprintf(stderr, "usage: ref_driver zouhe <in> <out>\n");
+    return rc;
+}

@@ -1,0 +1,85 @@
+"""Time-loop checks of the oracle (Simulation.cpp:903-932) that pin it to
+properties the reference's own step satisfies."""
+import numpy as np
+import pytest
+
+from lbm_cases import box_flags, pipe_case
+
+
+def closed_box(oracle, dims):
+    flags = box_flags(dims, (1, 1, 1), tuple(d - 1 for d in dims))
+    g = oracle.make_grid((0, 0, 0), 1.0, dims, (0, dims[0] - 1, 0, dims[1] - 1, 0, dims[2] - 1))
+    return oracle.classify(g, flags, [])
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_pure_streaming_shift(oracle, q):
+    """tau -> inf makes collision the identity; a tagged interior population
+    then moves exactly one lattice link per step (push == pull)."""
+    dims = (12, 11, 10)
+    flags = closed_box(oracle, dims)
+    c, w, _ = oracle.lattice(q)
+    s = oracle.Sim(q, dims, flags, 1e300, [], quirks=1)
+    f = s.f()
+    x0 = np.array([5, 5, 5])
+    site = lambda p: p[0] + dims[0] * (p[1] + dims[1] * p[2])
+    for d in range(q):
+        f[d, site(x0)] += 0.125 * (d + 1)
+    s.set_f(f)
+    s.step(2)
+    g = s.f()
+    for d in range(q):
+        assert g[d, site(x0 + 2 * c[d])] == w[d] + 0.125 * (d + 1)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_rest_state_is_fixed(oracle, q):
+    dims = (9, 8, 7)
+    flags = closed_box(oracle, dims)
+    _, w, _ = oracle.lattice(q)
+    s = oracle.Sim(q, dims, flags, 0.8, [], quirks=1)
+    s.step(50)
+    f = s.f()
+    fl = flags > 0
+    assert np.allclose(f[:, fl], w[:, None], rtol=2e-15, atol=0)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_mass_conserved_closed_box_clean(oracle, q):
+    dims = (10, 9, 8)
+    flags = closed_box(oracle, dims)
+    c, w, _ = oracle.lattice(q)
+    s = oracle.Sim(q, dims, flags, 0.7, [], quirks=0)
+    rng = np.random.default_rng(1)
+    f = s.f()
+    fl = flags > 0
+    f[:, fl] = w[:, None] * (1 + 0.02 * rng.standard_normal((q, fl.sum())))
+    s.set_f(f)
+    m0 = s.mass()
+    s.step(40)
+    assert abs(s.mass() - m0) < 1e-12 * m0
+
+
+def test_poiseuille_pipe(oracle):
+    """Steady pipe flow (clean bounce-back) is a parabola in r with a flux that
+    is the same at every cross-section (D3Q19).  The reference-quirk mode leaks
+    mass through the +x wall links it does not bounce back, so the check runs
+    on the clean mode."""
+    dims, R, umax = (14, 14, 40), 5.5, 0.02
+    cl, bcs = pipe_case(oracle, dims, R, umax)
+    s = oracle.Sim(19, dims, cl, 0.8, [oracle.make_bc(**b) for b in bcs], quirks=0)
+    s.step(900)
+    rho, u = s.macros()
+    nx, ny, nz = dims
+    uz = u[2].reshape(nz, ny, nx)
+    rr = rho.reshape(nz, ny, nx)
+    flux = [(rr[k] * uz[k]).sum() for k in (nz // 4, nz // 2, 3 * nz // 4)]
+    assert max(flux) - min(flux) < 5e-3 * max(flux)
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    r2 = ((i - (nx - 1) / 2) ** 2 + (j - (ny - 1) / 2) ** 2).astype(float)
+    inner = r2 < (R - 1.0) ** 2
+    prof = uz[nz // 2][inner]
+    A = np.stack([np.ones(inner.sum()), r2[inner]], 1)
+    (a, b), *_ = np.linalg.lstsq(A, prof, rcond=None)
+    assert np.linalg.norm(A @ [a, b] - prof) < 0.02 * np.linalg.norm(prof)
+    assert R - 1.0 < np.sqrt(-a / b) < R + 1.0

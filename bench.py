@@ -1,0 +1,179 @@
+"""Benchmark of the fused stream-collide step (MLUPS and HBM-roofline fraction).
+
+Workload (one GPU): D3Q19 BGK on a 256x256x512 lattice holding a seeded
+Y-bifurcation (parent radius 40 splitting at z=256 into two radius-30
+daughters at +-25 degrees, 8-mode wall perturbation of amplitude 1.5, about
+150k facets), voxelized on the GPU; Zou-He pulsatile parabolic inlet
+u(t) = 0.03 (1 + sum_k a_k sin(2 pi k t / 4000 + phi_k)), Zou-He pressure
+outlets rho = 1, half-way bounce-back walls, tau = 0.66.  The timed step
+stores populations in FP32 as f - w and collides in FP32.
+
+One JSON line: MLUPS over fluid nodes for the whole job, the roofline of the
+step kernel (algorithmic bytes 2*Q*s per fluid node per step over the average
+launch time measured with HIP events on the engine's stream), and the CPU
+oracle timed on a bounded sample of the same workload on this host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage f32|f64|f16s]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+TAU = 0.66
+SEED = 2013
+PERIOD = 4000.0
+
+
+def c2_case(P, ctx):
+    import numpy as np
+
+    from pv2013_amd.geometry import fourier_waveform, y_bifurcation
+    dims = (256, 256, 512)
+    tri, info = y_bifurcation(SEED, dims)
+    grid = P.make_grid((0.0, 0.0, 0.0), 1.0, dims, (0.0, dims[0] - 1.0, 0.0, dims[1] - 1.0, 0.0, dims[2] - 1.0))
+    flags, vox_ms = ctx.voxelize(tri, grid)
+    cx, cy = info["center"]
+    planes = [P.make_plane(2, "1"),
+              P.make_plane(2, str(info["outlet_z"]), ranges=((0, -1.0, cx),)),
+              P.make_plane(2, str(info["outlet_z"]), ranges=((0, cx, dims[0] + 1.0),))]
+    flags, cls_ms = ctx.classify(grid, flags, planes)
+    bcs = [dict(kind=P.BC_VELOCITY, axis=2, sigma=1, u=(0.0, 0.0, 0.03), profile=P.PROFILE_PARABOLIC,
+                center=(cx, cy), radius=info["parent_r"], waveform=0),
+           dict(kind=P.BC_PRESSURE, axis=2, sigma=-1, rho=1.0),
+           dict(kind=P.BC_PRESSURE, axis=2, sigma=-1, rho=1.0)]
+    a, phi = fourier_waveform(SEED)
+    return {"dims": dims, "flags": flags, "bcs": bcs, "wave": (a, phi), "ntri": len(tri),
+            "vox_ms": vox_ms, "cls_ms": cls_ms, "nsites": int(np.prod(dims))}
+
+
+def make_domain(P, ctx, case, storage):
+    dom = P.Domain(ctx, case["dims"], case["flags"], TAU, [P.make_bc(**b) for b in case["bcs"]],
+                   q=19, storage=storage)
+    a, phi = case["wave"]
+    dom.set_waveform(0, P.WAVE_FOURIER, PERIOD, a, phi)
+    return dom
+
+
+def run(P, ctx, case, storage, steps, warmup):
+    dom = make_domain(P, ctx, case, storage)
+    n = dom.info()["nodes"]
+    dom.step(warmup)
+    dom.sync()
+    t0 = time.perf_counter()
+    ms = dom.step_timed(steps)
+    wall = time.perf_counter() - t0
+    mass, _ = dom.reduce()
+    dom.close()
+    return {"nodes": n, "ms": ms, "wall_s": wall, "mass": mass}
+
+
+def cpu_baseline(case, budget_s=12.0):
+    """The FP64 oracle (a plain-C restatement of the reference step), one
+    thread, on the same domain for as many steps as fit in about budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    bcs = [O.make_bc(**b) for b in case["bcs"]]
+    sim = O.Sim(19, case["dims"], case["flags"], TAU, bcs, quirks=1)
+    a, phi = case["wave"]
+    sim.set_waveform(0, 1, PERIOD, a, phi)
+    t0 = time.perf_counter()
+    sim.step(1)
+    one = time.perf_counter() - t0
+    k = max(1, int(budget_s / max(one, 1e-6)) - 1)
+    t0 = time.perf_counter()
+    sim.step(k)
+    dt = time.perf_counter() - t0
+    return {"value": sim.nodes * k / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": "%d steps of the same 256x256x512 Y-bifurcation domain (%d fluid nodes), "
+                      "FP64 oracle, 1 thread" % (k, sim.nodes)}
+
+
+def traffic_from_profiles(storage_name):
+    path = os.path.join(ROOT, "profiles", "step_traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(storage_name)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f64", "f16s"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 / FP16 comparison runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the whole domain runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
+        return 2
+    import pv2013_amd as P
+    storage = {"f32": P.STORE_F32, "f64": P.STORE_F64, "f16s": P.STORE_F16S}[args.storage]
+    sbytes = {"f32": 4, "f64": 8, "f16s": 2}[args.storage]
+    ctx = P.Context(0)
+    print("bench: building the 256x256x512 Y-bifurcation case", file=sys.stderr, flush=True)
+    case = c2_case(P, ctx)
+    print("bench: timing %d steps (%s) after %d warm-up" % (args.steps, args.storage, args.warmup),
+          file=sys.stderr, flush=True)
+    r = run(P, ctx, case, storage, args.steps, args.warmup)
+    n = r["nodes"]
+    ms_step = r["ms"] / args.steps
+    mlups = n * args.steps / (r["ms"] * 1e-3) / 1e6
+    bytes_per_launch = n * 2 * 19 * sbytes
+    achieved = bytes_per_launch / (ms_step * 1e-3) / 1e9
+    extra = {"fluid_nodes": n, "box_sites": case["nsites"], "box_site_mlups": case["nsites"] / n * mlups,
+             "wall_s_timed": r["wall_s"], "mass_after": r["mass"], "facets": case["ntri"],
+             "voxelize_ms": case["vox_ms"], "classify_ms": case["cls_ms"],
+             "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3)}
+    if not args.no_compare:
+        for name, st in (("f64", P.STORE_F64), ("f16s", P.STORE_F16S)):
+            if name == args.storage:
+                continue
+            print("bench: comparison run %s" % name, file=sys.stderr, flush=True)
+            rr = run(P, ctx, case, st, min(args.steps, 1000), min(args.warmup, 100))
+            extra["mlups_" + name] = rr["nodes"] * min(args.steps, 1000) / (rr["ms"] * 1e-3) / 1e6
+        if "mlups_f64" in extra:
+            extra["speedup_vs_f64"] = mlups / extra["mlups_f64"]
+    cpu = None
+    if not args.no_cpu:
+        print("bench: CPU oracle sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(case)
+    line = {
+        "metric": "MLUPS & % HBM roofline, D3Q19 256^2x512; FP32/FP16 vs FP64 at stated error",
+        "value": mlups,
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "f32 (f-w storage), f32 collide", "f64": "f64",
+                  "f16s": "f16 (f-w storage), f32 collide"}[args.storage],
+        "data": "synthetic (seeded Y-bifurcation STL, seed %d)" % SEED,
+        "config": {"workload": "D3Q19 BGK, 256x256x512 Y-bifurcation, pulsatile Zou-He inlet, "
+                               "pressure outlets, bounce-back walls, tau 0.66",
+                   "lattice": "D3Q19", "box": [256, 256, 512], "storage": args.storage,
+                   "scheme": "two-lattice pull", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.storage)},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,176 @@
+/*
+ * lbm_capi.h - C ABI of the MI355X lattice-Boltzmann engine (libpv2013_lbm.so).
+ *
+ * Plain C types only.  Every call returns 0 on success or a negative errno-style
+ * code (LBM_E*); no C++ exception crosses this boundary.  Host buffers passed in
+ * are copied; the library owns all device memory and frees it in *_destroy.
+ * Calls on one lbm_ctx are stream-ordered and not thread-safe.
+ *
+ * The reference (Resacd/PV2013) has no FFI layer: its boundary is a handful of
+ * C++ classes constructed in main() (LBM.cpp:68-71).  Each entry point below
+ * names the reference code it replaces.
+ *
+ * Layouts shared by all calls:
+ *   - box arrays are x-fastest: site = i + nx*(j + ny*k);
+ *   - flags: 0 outside, 1 fluid, 2 wall, 3+b boundary condition b (inlet or
+ *     outlet, in the order given to lbm_classify / lbm_domain_create);
+ *   - triangles: 12 doubles each, v1 v2 v3 (xyz) then the normal as written in
+ *     the STL file (Struct.h:77-82);
+ *   - per-population arrays are [d][site], d in the reference's index order
+ *     (LatticeModel.cpp:51-106 for D3Q19, :208-266 for D3Q27).
+ */
+#ifndef PV2013_LBM_CAPI_H
+#define PV2013_LBM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_EIO (-5)      /* HIP runtime error, file error */
+#define LBM_ENOMEM (-12)  /* device or host allocation failed */
+#define LBM_ENODEV (-19)  /* no such device / library built without GPU support */
+#define LBM_EINVAL (-22)  /* invalid argument */
+#define LBM_ERANGE (-34)  /* size beyond a stated limit */
+#define LBM_ENAN (-61)    /* non-finite state found by a health check */
+
+typedef struct lbm_ctx lbm_ctx;
+typedef struct lbm_domain lbm_domain;
+
+/* ------------------------------------------------------------------ context */
+
+/* Opens `device` and creates the context stream.  Replaces the MPI_Init /
+ * cartesian communicator set-up (LBM.cpp:25, Subdomain.cpp:67-230): one GPU
+ * holds the whole domain. */
+int lbm_ctx_create(int device, lbm_ctx** out);
+void lbm_ctx_destroy(lbm_ctx* ctx);
+/* Text of the last error on this context (never NULL). */
+const char* lbm_last_error(const lbm_ctx* ctx);
+int lbm_version(void);
+
+/* ----------------------------------------------------------------- geometry */
+
+typedef struct {
+    double origin[3];  /* coordinate of grid point (0,0,0) */
+    double spacing;
+    int32_t dims[3];
+    int32_t pad_;
+    double raybox[6];  /* min_x, max_x, min_y, max_y, min_z, max_z seen by the ray */
+    double ray_offset; /* 0.01 */
+    double ray_pad;    /* 0.1 */
+    double t_eps;      /* 1e-7 */
+} lbm_grid_desc;
+
+/* Reader::readSTLASCII (GeometryReader.cpp:33-117).  Returns the facet count;
+ * fills at most `cap` facets when `tri` is not NULL. */
+int64_t lbm_read_stl_ascii(const char* path, double* tri, int64_t cap);
+/* Reader::readSTLBinary (GeometryReader.cpp:119-176), normals renormalised in
+ * float as the reference does. */
+int64_t lbm_read_stl_binary(const char* path, double* tri, int64_t cap);
+
+/* Geometry::GetBoundingbox + generatePointsInBoundingBox
+ * (ComplexGeometry.cpp:40-73, 178-207): origin = bbox min, dims =
+ * ceil(extent/spacing)+1, the ray box is the STL bbox. */
+int lbm_grid_from_bbox(const double* tri, int64_t ntri, double spacing, lbm_grid_desc* out);
+
+/* Geometry::labelPoints (ComplexGeometry.cpp:431-506): +z ray parity per grid
+ * point, bit-exact.  flags_out gets 0 (outside) or 1 (inside).  kernel_ms
+ * (may be NULL) receives the device time of the voxelizer kernels. */
+int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const lbm_grid_desc* grid,
+                 uint8_t* flags_out, double* kernel_ms);
+
+typedef struct {
+    int32_t axis;      /* coordinate tested against the plane value */
+    int32_t nranges;   /* 0..3 extra open-interval conditions */
+    double value_hi, value_lo; /* plane value, a long double split as hi + lo */
+    double tol;        /* |coord - value| <= tol, evaluated in long double */
+    int32_t range_axis[3];
+    int32_t pad_;
+    double range_lo[3], range_hi[3]; /* lo < coord < hi */
+} lbm_plane_desc;
+
+/* Geometry::findBoundaryPoints + getInOutBP + excludeInletOutlet
+ * (ComplexGeometry.cpp:509-638) and SubDomain::CreateNodes' priority
+ * (Subdomain.cpp:413-473).  flags in: 0/1; out: 0 outside, 1 fluid, 2 wall,
+ * 3+p for the first matching plane p. */
+int lbm_classify(lbm_ctx* ctx, const lbm_grid_desc* grid, const lbm_plane_desc* planes,
+                 int nplanes, uint8_t* flags_inout, double* kernel_ms);
+
+/* ------------------------------------------------------------------ domain */
+
+enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16S = 2 };
+enum { LBM_SCHEME_TWO_LATTICE = 0, LBM_SCHEME_AA = 1 };
+enum { LBM_BC_VELOCITY = 0, LBM_BC_PRESSURE = 1 };
+enum { LBM_PROFILE_UNIFORM = 0, LBM_PROFILE_PARABOLIC = 1 };
+enum { LBM_WAVE_NONE = 0, LBM_WAVE_FOURIER = 1, LBM_WAVE_PV2013 = 2 };
+
+typedef struct {
+    int32_t kind;     /* LBM_BC_VELOCITY | LBM_BC_PRESSURE (Zou-He) */
+    int32_t axis;     /* face normal axis */
+    int32_t sigma;    /* sign of c[axis] of the incoming (unknown) populations */
+    int32_t profile;  /* velocity: uniform or parabolic 1 - r^2/R^2 (clamped at 0) */
+    double rho;       /* pressure BC density */
+    double u[3];      /* velocity BC vector (scaled by profile and waveform) */
+    double center[2]; /* profile centre on the two tangential axes (lattice units) */
+    double radius;
+    int32_t waveform; /* -1 none, else a waveform slot set by lbm_set_waveform */
+    int32_t typo;     /* reproduce the x-face N_xz f10/f16 swap (Simulation.cpp:1124) */
+} lbm_bc_desc;
+
+typedef struct {
+    int32_t dims[3];
+    int32_t q;        /* 19 or 27 */
+    int32_t storage;  /* LBM_STORE_* */
+    int32_t scheme;   /* LBM_SCHEME_* */
+    int32_t quirks;   /* 1: the reference's step semantics; 0: clean bounce-back */
+    int32_t block;    /* threads per block, 0 = default */
+    double tau;
+    const uint8_t* flags;    /* host, dims product */
+    const lbm_bc_desc* bcs;  /* bcs[b] drives flag 3+b */
+    int32_t nbc;
+    int32_t fast_axis;       /* node order: fastest-varying axis, -1 = longest runs */
+} lbm_domain_desc;
+
+/* SubDomain(Geometry&) (Subdomain.cpp:3-33: CreateNodes, Connectivity,
+ * Initialize, GetFluidHaloLat) + Simulation's constructor set-up
+ * (Simulation.cpp:3-12): builds the packed fluid-node list, site map,
+ * boundary records, and initialises f = w (Node.cpp:30-37). */
+int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_domain** out);
+void lbm_domain_destroy(lbm_domain* dom);
+int lbm_domain_info(const lbm_domain* dom, int64_t* n_nodes, int64_t* n_boundary,
+                    int64_t* device_bytes);
+
+/* Pulsatile inlet waveform for slot (Simulation.cpp:875-900, pulsatile.h).
+ * FOURIER: W(t) = 1 + sum_k a_k sin(2 pi k t / period + phi_k), t in steps.
+ * PV2013: the reference's piecewise polynomial at t*dt seconds, period 0.9. */
+int lbm_set_waveform(lbm_domain* dom, int slot, int kind, double period, int nk,
+                     const double* a, const double* phi, double dt);
+
+/* Simulation::Iterate body (Simulation.cpp:908-918) nsteps times: collision,
+ * streaming, inlet, wall, outlet.  Enqueued on the context stream. */
+int lbm_step(lbm_domain* dom, int64_t nsteps);
+/* Same, and returns the device time of the steps (HIP events on the stream). */
+int lbm_step_timed(lbm_domain* dom, int64_t nsteps, double* ms);
+int lbm_sync(lbm_domain* dom);
+int64_t lbm_iteration(const lbm_domain* dom);
+
+/* State after the last step, FP64, dense [d][site] (zero outside). */
+int lbm_get_f(lbm_domain* dom, double* f);
+/* Replace the state (dense [d][site]); the reference's stale populations take
+ * these values. */
+int lbm_set_f(lbm_domain* dom, const double* f);
+/* Moments of the current state (Simulation.cpp:32-78): rho and u [c][site]. */
+int lbm_get_macros(lbm_domain* dom, double* rho, double* u);
+/* Wall shear stress of the current state on wall sites
+ * (Simulation.cpp:1374-1421), zero elsewhere. */
+int lbm_get_wss(lbm_domain* dom, double* wss);
+/* FP64 health check: total mass and per-boundary-condition mass flux through
+ * its face (flux has nbc entries, may be NULL).  LBM_ENAN on a non-finite state. */
+int lbm_reduce(lbm_domain* dom, double* mass, double* flux);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

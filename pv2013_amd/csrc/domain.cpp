@@ -1,0 +1,719 @@
+// Host side of the engine: context, domain build (packed node list, padded
+// site map, boundary records), time stepping and the read-back entry points
+// of include/lbm_capi.h.
+//
+// Domain build replaces SubDomain::CreateNodes / Connectivity / Initialize /
+// GetFluidHaloLat (Subdomain.cpp:375-651):
+//   - nodes are the non-zero flags, numbered with the fastest axis along the
+//     longest fluid runs (the vessel direction) so a wavefront of 64 nodes is a
+//     contiguous run and its pulls from neighbouring rows are coalesced;
+//   - a padded int32 site map (one layer of -1 around the box, in the same axis
+//     order) turns the reference's neighbour pointers (Node::m_neighbours) into
+//     node = map[p(x) - off(c_d)] with no bounds checks;
+//   - nodes with a missing link or a boundary condition get a record (found
+//     through a bitmap + per-word prefix count): which populations bounce back,
+//     the stale values the reference leaves in unlinked populations, and the
+//     velocity profile factor.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lattice.h"
+#include "lbm_internal.h"
+
+struct lbm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+int lbm_ctx_fail(lbm_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+hipStream_t lbm_ctx_stream(lbm_ctx* ctx) { return ctx->stream; }
+int lbm_ctx_device(lbm_ctx* ctx) { return ctx->device; }
+
+#define CHECK(call)                                                                             \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return lbm_ctx_fail(ctx, LBM_EIO, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int lbm_version(void) { return 1; }
+
+extern "C" int lbm_ctx_create(int device, lbm_ctx** out) {
+    if (!out) return LBM_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return LBM_ENODEV;
+    if (device < 0 || device >= n) return LBM_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return LBM_ENODEV;
+    lbm_ctx* c = new lbm_ctx;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return LBM_EIO;
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+extern "C" void lbm_ctx_destroy(lbm_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char* lbm_last_error(const lbm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// ------------------------------------------------------------------ domain
+
+struct lbm_domain {
+    lbm_ctx* ctx = nullptr;
+    int q = 19, storage = 0, scheme = 0, quirks = 1;
+    double tau = 1.0;
+    int dims[3] = {0, 0, 0};
+    int perm[3] = {0, 1, 2};  // perm[0] is the fastest-varying axis of the node order
+    int pd[3] = {0, 0, 0};    // padded dims in permuted order
+    int64_t nsites = 0, n = 0, nrec = 0;
+    std::vector<int64_t> node_site;  // host: node -> x-fastest site
+    std::vector<int32_t> wall_nodes;
+    std::vector<int32_t> bc_nodes, bc_face;
+    std::vector<FaceTab> faces;
+    std::vector<WaveTab> waves;
+    std::vector<void*> allocs;
+    int64_t device_bytes = 0;
+    // device arrays
+    int32_t *pnode = nullptr, *map = nullptr, *bpre = nullptr;
+    uint64_t* bbits = nullptr;
+    uint8_t* btype = nullptr;
+    uint32_t* bbb = nullptr;
+    double* bprof = nullptr;
+    void* bstale = nullptr;
+    FaceTab* d_faces = nullptr;
+    WaveTab* d_waves = nullptr;
+    int64_t* d_iter = nullptr;
+    uint32_t* d_ticket = nullptr;
+    void* g[2] = {nullptr, nullptr};
+    int32_t *d_wall = nullptr, *d_bcnode = nullptr, *d_bcface = nullptr;
+    double *fpre = nullptr, *rho = nullptr, *u = nullptr, *wss = nullptr, *red = nullptr;
+    int cur = 0;
+    int64_t iter = 0;
+    int state = 0;            // 0: f = w (initial), 1: fpre holds f, 2: recompute from g[cur ^ 1]
+    int64_t pre_iter = -1;    // iteration for which fpre/rho/u are valid
+    int32_t off[27];
+    int block = 256;
+
+    template <class T>
+    hipError_t alloc(T** p, size_t count) {
+        void* v = nullptr;
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&v, bytes);
+        if (e == hipSuccess) {
+            allocs.push_back(v);
+            device_bytes += (int64_t)bytes;
+        }
+        *p = (T*)v;
+        return e;
+    }
+    ~lbm_domain() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
+static size_t store_bytes(int storage) { return storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2); }
+
+static int choose_fast_axis(const uint8_t* flags, const int* dims) {
+    // fewest runs per fluid site = longest runs
+    int64_t runs[3] = {0, 0, 0};
+    const int nx = dims[0], ny = dims[1], nz = dims[2];
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t s = i + (int64_t)nx * (j + (int64_t)ny * k);
+                if (!flags[s]) continue;
+                if (i == 0 || !flags[s - 1]) runs[0]++;
+                if (j == 0 || !flags[s - nx]) runs[1]++;
+                if (k == 0 || !flags[s - (int64_t)nx * ny]) runs[2]++;
+            }
+    int best = 2;
+    for (int a = 1; a >= 0; --a)
+        if (runs[a] < runs[best]) best = a;
+    return best;
+}
+
+static FaceTab make_face(const lbm_bc_desc& b) {
+    FaceTab F;
+    std::memset(&F, 0, sizeof F);
+    F.kind = b.kind;
+    F.axis = b.axis;
+    F.sigma = b.sigma;
+    F.waveform = b.waveform;
+    F.typo = b.typo;
+    F.rho = b.rho;
+    F.u[0] = b.u[0];
+    F.u[1] = b.u[1];
+    F.u[2] = b.u[2];
+    return F;
+}
+
+// Parabolic factor 1 - r^2/R^2 (clamped at 0), r measured from the profile
+// centre on the two tangential axes in lattice units.
+static double profile_factor(const lbm_bc_desc& b, const int* ijk) {
+    if (b.kind != LBM_BC_VELOCITY || b.profile != LBM_PROFILE_PARABOLIC) return 1.0;
+    int ta = -1, tb = -1;
+    for (int x = 0; x < 3; ++x)
+        if (x != b.axis) {
+            if (ta < 0) ta = x;
+            else tb = x;
+        }
+    const double d1 = ijk[ta] - b.center[0];
+    const double d2 = ijk[tb] - b.center[1];
+    const double r2 = d1 * d1 + d2 * d2;
+    double pf = 1.0 - r2 / (b.radius * b.radius);
+    if (pf < 0.0) pf = 0.0;
+    return pf;
+}
+
+static int upload(lbm_domain* D, void* dst, const void* src, size_t bytes) {
+    lbm_ctx* ctx = D->ctx;
+    if (bytes) CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return LBM_OK;
+}
+
+// Stale populations of boundary records, [e][r] in the storage's calc type.
+static int upload_stale(lbm_domain* D, const std::vector<double>& stale_f64) {
+    if (D->storage == LBM_STORE_F64) return upload(D, D->bstale, stale_f64.data(), sizeof(double) * stale_f64.size());
+    std::vector<float> dev(stale_f64.size());
+    for (int e = 0; e < D->q; ++e)
+        for (int64_t r = 0; r < D->nrec; ++r)
+            dev[e * D->nrec + r] = (float)(stale_f64[e * D->nrec + r] - lat_w(D->q, e));
+    int rc = upload(D, D->bstale, dev.data(), sizeof(float) * dev.size());
+    if (rc) return rc;
+    lbm_ctx* ctx = D->ctx;
+    CHECK(hipStreamSynchronize(ctx->stream));
+    return LBM_OK;
+}
+
+static void fill_args(const lbm_domain* D, StepArgs& a) {
+    std::memset(&a, 0, sizeof a);
+    a.n = D->n;
+    a.nrec = D->nrec;
+    a.q = D->q;
+    a.pnode = D->pnode;
+    a.map = D->map;
+    a.bbits = D->bbits;
+    a.bpre = D->bpre;
+    a.btype = D->btype;
+    a.bbb = D->bbb;
+    a.bprof = D->bprof;
+    a.bstale = D->bstale;
+    a.faces = D->d_faces;
+    a.waves = D->d_waves;
+    a.iter = D->d_iter;
+    a.ticket = D->d_ticket;
+    a.omega = 1.0 / D->tau;
+    for (int d = 0; d < 27; ++d) a.off[d] = D->off[d];
+}
+
+static int launch_step(lbm_domain* D, int mode, const void* gin, void* gout) {
+    lbm_ctx* ctx = D->ctx;
+    StepArgs a;
+    fill_args(D, a);
+    a.gin = gin;
+    a.gout = gout;
+    a.fpre = D->fpre;
+    a.rho = D->rho;
+    a.u = D->u;
+    hipError_t e = D->storage == LBM_STORE_F64 ? launch_step_f64(mode, a, D->block, ctx->stream)
+                                               : launch_step_lp(D->storage, mode, a, D->block, ctx->stream);
+    CHECK(e);
+    return LBM_OK;
+}
+
+static int init_from(lbm_domain* D, const double* fpre_dev) {
+    lbm_ctx* ctx = D->ctx;
+    InitArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.n = D->n;
+    a.fpre = fpre_dev;
+    a.gout = D->g[D->cur];
+    a.omega = 1.0 / D->tau;
+    hipError_t e = D->storage == LBM_STORE_F64 ? launch_init_f64(D->q, a, D->block, ctx->stream)
+                                               : launch_init_lp(D->storage, D->q, a, D->block, ctx->stream);
+    CHECK(e);
+    return LBM_OK;
+}
+
+static int ensure_pre_buffers(lbm_domain* D) {
+    lbm_ctx* ctx = D->ctx;
+    if (!D->fpre) CHECK(D->alloc(&D->fpre, (size_t)D->q * D->n));
+    if (!D->rho) CHECK(D->alloc(&D->rho, (size_t)D->n));
+    if (!D->u) CHECK(D->alloc(&D->u, (size_t)3 * D->n));
+    return LBM_OK;
+}
+
+extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_domain** out) {
+    if (!ctx) return LBM_EINVAL;
+    if (!desc || !out || !desc->flags) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_domain_create: null argument");
+    *out = nullptr;
+    const lbm_domain_desc& d = *desc;
+    if (d.q != 19 && d.q != 27) return lbm_ctx_fail(ctx, LBM_EINVAL, "q must be 19 or 27");
+    if (d.storage < 0 || d.storage > 2) return lbm_ctx_fail(ctx, LBM_EINVAL, "bad storage");
+    if (d.scheme != LBM_SCHEME_TWO_LATTICE) return lbm_ctx_fail(ctx, LBM_EINVAL, "scheme not supported");
+    if (!(d.tau > 0.5)) return lbm_ctx_fail(ctx, LBM_EINVAL, "tau must exceed 0.5");
+    for (int a = 0; a < 3; ++a)
+        if (d.dims[a] <= 0) return lbm_ctx_fail(ctx, LBM_EINVAL, "dims must be positive");
+    if (d.nbc < 0 || d.nbc > LBM_MAXBC || (d.nbc > 0 && !d.bcs))
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "bad boundary-condition list");
+    for (int b = 0; b < d.nbc; ++b) {
+        const lbm_bc_desc& c = d.bcs[b];
+        if ((c.kind != LBM_BC_VELOCITY && c.kind != LBM_BC_PRESSURE) || c.axis < 0 || c.axis > 2 ||
+            (c.sigma != 1 && c.sigma != -1) || c.waveform >= LBM_MAXWAVE ||
+            (c.kind == LBM_BC_PRESSURE && !(c.rho > 0)) ||
+            (c.profile == LBM_PROFILE_PARABOLIC && !(c.radius > 0)))
+            return lbm_ctx_fail(ctx, LBM_EINVAL, "bad boundary condition " + std::to_string(b));
+    }
+    CHECK(hipSetDevice(ctx->device));
+    lbm_domain* D = new lbm_domain;
+    D->ctx = ctx;
+    D->q = d.q;
+    D->storage = d.storage;
+    D->scheme = d.scheme;
+    D->quirks = d.quirks;
+    D->tau = d.tau;
+    D->block = d.block > 0 ? d.block : 256;
+    for (int a = 0; a < 3; ++a) D->dims[a] = d.dims[a];
+    const int nx = d.dims[0], ny = d.dims[1], nz = d.dims[2];
+    D->nsites = (int64_t)nx * ny * nz;
+    const uint8_t* flags = d.flags;
+    for (int64_t s = 0; s < D->nsites; ++s)
+        if (flags[s] >= 3 && flags[s] - 3 >= d.nbc) {
+            delete D;
+            return lbm_ctx_fail(ctx, LBM_EINVAL, "flag refers to a missing boundary condition");
+        }
+    const int fast = (d.fast_axis >= 0 && d.fast_axis < 3) ? d.fast_axis : choose_fast_axis(flags, d.dims);
+    D->perm[0] = fast;
+    for (int a = 0, k = 1; a < 3; ++a)
+        if (a != fast) D->perm[k++] = a;
+    for (int k = 0; k < 3; ++k) D->pd[k] = d.dims[D->perm[k]] + 2;
+    const int64_t mapn = (int64_t)D->pd[0] * D->pd[1] * D->pd[2];
+    if (mapn >= (int64_t)INT32_MAX) {
+        delete D;
+        return lbm_ctx_fail(ctx, LBM_ERANGE, "box too large for the int32 site map");
+    }
+    const int q = d.q;
+    const int rest = lat_rest(q);
+    for (int e = 0; e < 27; ++e) D->off[e] = 0;
+    for (int e = 0; e < q; ++e) {
+        const int c0 = lat_c(q, e, D->perm[0]), c1 = lat_c(q, e, D->perm[1]), c2 = lat_c(q, e, D->perm[2]);
+        D->off[e] = c0 + D->pd[0] * (c1 + D->pd[1] * c2);
+    }
+    // node numbering and padded map
+    std::vector<int32_t> map((size_t)mapn, -1);
+    std::vector<int32_t> pnode;
+    std::vector<int64_t>& site = D->node_site;
+    int ext[3] = {d.dims[D->perm[0]], d.dims[D->perm[1]], d.dims[D->perm[2]]};
+    int ijk[3];
+    for (int s2 = 0; s2 < ext[2]; ++s2)
+        for (int s1 = 0; s1 < ext[1]; ++s1)
+            for (int s0 = 0; s0 < ext[0]; ++s0) {
+                ijk[D->perm[0]] = s0;
+                ijk[D->perm[1]] = s1;
+                ijk[D->perm[2]] = s2;
+                const int64_t st = ijk[0] + (int64_t)nx * (ijk[1] + (int64_t)ny * ijk[2]);
+                if (!flags[st]) continue;
+                const int64_t p = (s0 + 1) + (int64_t)D->pd[0] * ((s1 + 1) + (int64_t)D->pd[1] * (s2 + 1));
+                map[p] = (int32_t)site.size();
+                pnode.push_back((int32_t)p);
+                site.push_back(st);
+            }
+    D->n = (int64_t)site.size();
+    if (D->n >= (int64_t)INT32_MAX / 2) {
+        delete D;
+        return lbm_ctx_fail(ctx, LBM_ERANGE, "too many fluid nodes");
+    }
+    if (D->n == 0) {
+        delete D;
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "no fluid nodes");
+    }
+    // boundary records
+    const int64_t nwords = (D->n + 63) / 64;
+    std::vector<uint64_t> bits((size_t)nwords, 0);
+    std::vector<int32_t> pre((size_t)nwords, 0);
+    std::vector<uint8_t> btype;
+    std::vector<uint32_t> bbb;
+    std::vector<double> bprof;
+    std::vector<uint32_t> bstale_mask;
+    D->faces.resize(std::max(d.nbc, 1));
+    for (int b = 0; b < d.nbc; ++b) D->faces[b] = make_face(d.bcs[b]);
+    D->waves.assign(LBM_MAXWAVE, WaveTab());
+    for (auto& w : D->waves) std::memset(&w, 0, sizeof w);
+    for (int64_t m = 0; m < D->n; ++m) {
+        const int64_t st = site[m];
+        ijk[0] = (int)(st % nx);
+        ijk[1] = (int)((st / nx) % ny);
+        ijk[2] = (int)(st / ((int64_t)nx * ny));
+        const int type = flags[st];
+        uint32_t miss = 0;
+        for (int e = 0; e < q; ++e) {
+            if (e == rest) continue;
+            const int x = ijk[0] - lat_c(q, e, 0), y = ijk[1] - lat_c(q, e, 1), z = ijk[2] - lat_c(q, e, 2);
+            const bool in = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
+            if (!in || !flags[x + (int64_t)nx * (y + (int64_t)ny * z)]) miss |= 1u << e;
+        }
+        if (type == 2) D->wall_nodes.push_back((int32_t)m);
+        if (type >= 3) {
+            D->bc_nodes.push_back((int32_t)m);
+            D->bc_face.push_back(type - 3);
+        }
+        if (type < 3 && miss == 0) continue;
+        bits[m >> 6] |= 1ull << (m & 63);
+        uint32_t bb;
+        if (D->quirks) {
+            // Simulation.cpp:1353-1370: only wall nodes bounce back, directions
+            // 1..q-1 (for D3Q19 direction 0, the +x link, is never bounced)
+            bb = type == 2 ? (miss & ~(q == 19 ? (1u << lat_opp(q, 0)) : 0u)) : 0u;
+        } else {
+            bb = miss;
+        }
+        btype.push_back((uint8_t)type);
+        bbb.push_back(bb);
+        bstale_mask.push_back(miss & ~bb);
+        bprof.push_back(type >= 3 ? profile_factor(d.bcs[type - 3], ijk) : 1.0);
+    }
+    D->nrec = (int64_t)btype.size();
+    int32_t acc = 0;
+    for (int64_t w = 0; w < nwords; ++w) {
+        pre[w] = acc;
+        acc += __builtin_popcountll(bits[w]);
+    }
+    // device allocation
+    int rc = LBM_OK;
+    auto fail = [&](int code) {
+        delete D;
+        return code;
+    };
+#define DCHECK(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) {                                                                      \
+            lbm_ctx_fail(ctx, e_ == hipErrorOutOfMemory ? LBM_ENOMEM : LBM_EIO,                     \
+                         std::string(#call) + ": " + hipGetErrorString(e_));                        \
+            return fail(e_ == hipErrorOutOfMemory ? LBM_ENOMEM : LBM_EIO);                          \
+        }                                                                                            \
+    } while (0)
+    const size_t sb = store_bytes(d.storage);
+    const size_t stale_sb = d.storage == LBM_STORE_F64 ? 8 : 4;
+    DCHECK(D->alloc(&D->pnode, (size_t)D->n));
+    DCHECK(D->alloc(&D->map, (size_t)mapn));
+    DCHECK(D->alloc(&D->bbits, (size_t)nwords));
+    DCHECK(D->alloc(&D->bpre, (size_t)nwords));
+    DCHECK(D->alloc(&D->btype, (size_t)D->nrec));
+    DCHECK(D->alloc(&D->bbb, (size_t)D->nrec));
+    DCHECK(D->alloc(&D->bprof, (size_t)D->nrec));
+    uint8_t* stale_raw = nullptr;
+    DCHECK(D->alloc(&stale_raw, (size_t)q * std::max<int64_t>(D->nrec, 1) * stale_sb));
+    D->bstale = stale_raw;
+    DCHECK(D->alloc(&D->d_faces, D->faces.size()));
+    DCHECK(D->alloc(&D->d_waves, D->waves.size()));
+    DCHECK(D->alloc(&D->d_iter, 1));
+    DCHECK(D->alloc(&D->d_ticket, 1));
+    uint8_t *g0 = nullptr, *g1 = nullptr;
+    DCHECK(D->alloc(&g0, (size_t)q * D->n * sb));
+    DCHECK(D->alloc(&g1, (size_t)q * D->n * sb));
+    D->g[0] = g0;
+    D->g[1] = g1;
+    DCHECK(D->alloc(&D->d_wall, std::max<size_t>(D->wall_nodes.size(), 1)));
+    DCHECK(D->alloc(&D->d_bcnode, std::max<size_t>(D->bc_nodes.size(), 1)));
+    DCHECK(D->alloc(&D->d_bcface, std::max<size_t>(D->bc_face.size(), 1)));
+    DCHECK(D->alloc(&D->red, (size_t)LBM_MAXBC + 2));
+    if ((rc = upload(D, D->pnode, pnode.data(), sizeof(int32_t) * pnode.size()))) return fail(rc);
+    if ((rc = upload(D, D->map, map.data(), sizeof(int32_t) * map.size()))) return fail(rc);
+    if ((rc = upload(D, D->bbits, bits.data(), sizeof(uint64_t) * bits.size()))) return fail(rc);
+    if ((rc = upload(D, D->bpre, pre.data(), sizeof(int32_t) * pre.size()))) return fail(rc);
+    if ((rc = upload(D, D->btype, btype.data(), btype.size()))) return fail(rc);
+    if ((rc = upload(D, D->bbb, bbb.data(), sizeof(uint32_t) * bbb.size()))) return fail(rc);
+    if ((rc = upload(D, D->bprof, bprof.data(), sizeof(double) * bprof.size()))) return fail(rc);
+    if ((rc = upload(D, D->d_faces, D->faces.data(), sizeof(FaceTab) * D->faces.size()))) return fail(rc);
+    if ((rc = upload(D, D->d_waves, D->waves.data(), sizeof(WaveTab) * D->waves.size()))) return fail(rc);
+    if ((rc = upload(D, D->d_wall, D->wall_nodes.data(), sizeof(int32_t) * D->wall_nodes.size()))) return fail(rc);
+    if ((rc = upload(D, D->d_bcnode, D->bc_nodes.data(), sizeof(int32_t) * D->bc_nodes.size()))) return fail(rc);
+    if ((rc = upload(D, D->d_bcface, D->bc_face.data(), sizeof(int32_t) * D->bc_face.size()))) return fail(rc);
+    const int64_t zero64 = 0;
+    const uint32_t zero32 = 0;
+    if ((rc = upload(D, D->d_iter, &zero64, sizeof zero64))) return fail(rc);
+    if ((rc = upload(D, D->d_ticket, &zero32, sizeof zero32))) return fail(rc);
+    // Node.cpp:30-37: f = w everywhere, so the stale populations hold w
+    std::vector<double> stale((size_t)q * std::max<int64_t>(D->nrec, 1), 0.0);
+    for (int e = 0; e < q; ++e)
+        for (int64_t r = 0; r < D->nrec; ++r) stale[e * D->nrec + r] = lat_w(q, e);
+    if ((rc = upload_stale(D, stale))) return fail(rc);
+    D->cur = 0;
+    if ((rc = init_from(D, nullptr))) return fail(rc);
+    DCHECK(hipStreamSynchronize(ctx->stream));
+#undef DCHECK
+    D->state = 0;
+    *out = D;
+    return LBM_OK;
+}
+
+extern "C" void lbm_domain_destroy(lbm_domain* D) {
+    if (!D) return;
+    (void)hipSetDevice(D->ctx->device);
+    (void)hipStreamSynchronize(D->ctx->stream);
+    delete D;
+}
+
+extern "C" int lbm_domain_info(const lbm_domain* D, int64_t* n_nodes, int64_t* n_boundary, int64_t* device_bytes) {
+    if (!D) return LBM_EINVAL;
+    if (n_nodes) *n_nodes = D->n;
+    if (n_boundary) *n_boundary = D->nrec;
+    if (device_bytes) *device_bytes = D->device_bytes;
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_waveform(lbm_domain* D, int slot, int kind, double period, int nk, const double* a,
+                                const double* phi, double dt) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    if (slot < 0 || slot >= LBM_MAXWAVE || nk < 0 || nk > LBM_MAXK || (nk > 0 && (!a || !phi)) ||
+        kind < 0 || kind > 2 || (kind != LBM_WAVE_NONE && !(period > 0)))
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_set_waveform: bad argument");
+    CHECK(hipSetDevice(ctx->device));
+    WaveTab& w = D->waves[slot];
+    std::memset(&w, 0, sizeof w);
+    w.kind = kind;
+    w.period = period;
+    w.dt = dt;
+    w.nk = kind == LBM_WAVE_FOURIER ? nk : 0;
+    for (int k = 0; k < w.nk; ++k) {
+        w.a[k] = a[k];
+        w.phi[k] = phi[k];
+    }
+    CHECK(hipMemcpyAsync(&D->d_waves[slot], &w, sizeof w, hipMemcpyHostToDevice, ctx->stream));
+    CHECK(hipStreamSynchronize(ctx->stream));
+    D->pre_iter = -1;
+    return LBM_OK;
+}
+
+static int step_n(lbm_domain* D, int64_t nsteps) {
+    for (int64_t i = 0; i < nsteps; ++i) {
+        int rc = launch_step(D, LBM_MODE_STEP, D->g[D->cur], D->g[D->cur ^ 1]);
+        if (rc) return rc;
+        D->cur ^= 1;
+        D->iter++;
+    }
+    if (nsteps > 0) {
+        D->state = 2;
+        D->pre_iter = -1;
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_step(lbm_domain* D, int64_t nsteps) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    if (nsteps < 0) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_step: negative step count");
+    CHECK(hipSetDevice(ctx->device));
+    return step_n(D, nsteps);
+}
+
+extern "C" int lbm_step_timed(lbm_domain* D, int64_t nsteps, double* ms) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    if (nsteps < 0) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_step_timed: negative step count");
+    CHECK(hipSetDevice(ctx->device));
+    CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+    int rc = step_n(D, nsteps);
+    if (rc) return rc;
+    CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+    CHECK(hipEventSynchronize(ctx->ev1));
+    float t = 0.f;
+    CHECK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (ms) *ms = t;
+    return LBM_OK;
+}
+
+extern "C" int lbm_sync(lbm_domain* D) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    CHECK(hipSetDevice(ctx->device));
+    CHECK(hipStreamSynchronize(ctx->stream));
+    return LBM_OK;
+}
+
+extern "C" int64_t lbm_iteration(const lbm_domain* D) { return D ? D->iter : -1; }
+
+// f, rho and u of the current state on the device (fpre, rho, u).
+__global__ void k_fill_rest(double* fpre, int64_t n, int q) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int d = 0; d < q; ++d) fpre[(int64_t)d * n + i] = q == 19 ? Lat<19>::w(d) : Lat<27>::w(d);
+}
+
+template <int Q>
+__global__ void k_moments_from_f(const double* fpre, int64_t n, double* rho, double* u) {
+    using L = Lat<Q>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double f[Q];
+    for (int d = 0; d < Q; ++d) f[d] = fpre[(int64_t)d * n + i];
+    double r = f[0];
+    for (int d = 1; d < Q; ++d) r = r + f[d];
+    for (int a = 0; a < 3; ++a) {
+        double p = 0.0, m = 0.0;
+        bool fp = true, fm = true;
+        for (int d = 0; d < Q; ++d) {
+            if (L::c(d, a) == 1) { p = fp ? f[d] : p + f[d]; fp = false; }
+            if (L::c(d, a) == -1) { m = fm ? f[d] : m + f[d]; fm = false; }
+        }
+        u[(int64_t)a * n + i] = (p - m) / r;
+    }
+    rho[i] = r;
+}
+
+static int ensure_pre(lbm_domain* D) {
+    lbm_ctx* ctx = D->ctx;
+    if (D->pre_iter == D->iter) return LBM_OK;
+    int rc = ensure_pre_buffers(D);
+    if (rc) return rc;
+    const unsigned grid = (unsigned)((D->n + 255) / 256);
+    if (D->state == 2) {
+        rc = launch_step(D, LBM_MODE_PRE, D->g[D->cur ^ 1], nullptr);
+        if (rc) return rc;
+    } else {
+        if (D->state == 0) hipLaunchKernelGGL(k_fill_rest, dim3(grid), dim3(256), 0, ctx->stream, D->fpre, D->n, D->q);
+        if (D->q == 19)
+            hipLaunchKernelGGL(k_moments_from_f<19>, dim3(grid), dim3(256), 0, ctx->stream, D->fpre, D->n, D->rho, D->u);
+        else
+            hipLaunchKernelGGL(k_moments_from_f<27>, dim3(grid), dim3(256), 0, ctx->stream, D->fpre, D->n, D->rho, D->u);
+        CHECK(hipGetLastError());
+    }
+    CHECK(hipStreamSynchronize(ctx->stream));
+    D->pre_iter = D->iter;
+    return LBM_OK;
+}
+
+static int scatter_dense(lbm_domain* D, const double* dev, int ncomp, double* out) {
+    lbm_ctx* ctx = D->ctx;
+    std::vector<double> h((size_t)ncomp * D->n);
+    CHECK(hipMemcpy(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(double) * ncomp * D->nsites);
+    for (int c = 0; c < ncomp; ++c)
+        for (int64_t m = 0; m < D->n; ++m) out[c * D->nsites + D->node_site[m]] = h[c * D->n + m];
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_f(lbm_domain* D, double* f) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    if (!f) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_get_f: null output");
+    CHECK(hipSetDevice(ctx->device));
+    int rc = ensure_pre(D);
+    if (rc) return rc;
+    return scatter_dense(D, D->fpre, D->q, f);
+}
+
+extern "C" int lbm_get_macros(lbm_domain* D, double* rho, double* u) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    CHECK(hipSetDevice(ctx->device));
+    int rc = ensure_pre(D);
+    if (rc) return rc;
+    if (rho && (rc = scatter_dense(D, D->rho, 1, rho))) return rc;
+    if (u && (rc = scatter_dense(D, D->u, 3, u))) return rc;
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_f(lbm_domain* D, const double* f) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    if (!f) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_set_f: null input");
+    CHECK(hipSetDevice(ctx->device));
+    int rc = ensure_pre_buffers(D);
+    if (rc) return rc;
+    const int q = D->q;
+    std::vector<double> h((size_t)q * D->n);
+    for (int e = 0; e < q; ++e)
+        for (int64_t m = 0; m < D->n; ++m) h[e * D->n + m] = f[e * D->nsites + D->node_site[m]];
+    CHECK(hipMemcpy(D->fpre, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    // stale populations take the new values (the reference never rewrites them)
+    std::vector<double> stale((size_t)q * std::max<int64_t>(D->nrec, 1), 0.0);
+    {
+        const int64_t nwords = (D->n + 63) / 64;
+        std::vector<uint64_t> bits((size_t)nwords);
+        CHECK(hipMemcpy(bits.data(), D->bbits, sizeof(uint64_t) * nwords, hipMemcpyDeviceToHost));
+        int64_t r = 0;
+        for (int64_t m = 0; m < D->n; ++m)
+            if ((bits[m >> 6] >> (m & 63)) & 1ull) {
+                for (int e = 0; e < q; ++e) stale[e * D->nrec + r] = h[e * D->n + m];
+                ++r;
+            }
+    }
+    if ((rc = upload_stale(D, stale))) return rc;
+    if ((rc = init_from(D, D->fpre))) return rc;
+    CHECK(hipStreamSynchronize(ctx->stream));
+    D->state = 1;
+    D->pre_iter = -1;
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_wss(lbm_domain* D, double* wss) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    if (!wss) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_get_wss: null output");
+    CHECK(hipSetDevice(ctx->device));
+    int rc = ensure_pre(D);
+    if (rc) return rc;
+    if (!D->wss) CHECK(D->alloc(&D->wss, std::max<size_t>(D->wall_nodes.size(), 1)));
+    WssArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.nwall = (int64_t)D->wall_nodes.size();
+    a.n = D->n;
+    a.wall = D->d_wall;
+    a.pnode = D->pnode;
+    a.map = D->map;
+    a.u = D->u;
+    a.wss = D->wss;
+    a.mu = 1.0 / 6.0 * (2.0 * D->tau - 1.0);
+    for (int e = 0; e < 27; ++e) a.off[e] = D->off[e];
+    CHECK(launch_wss(D->q, a, ctx->stream));
+    std::vector<double> h(std::max<size_t>(D->wall_nodes.size(), 1));
+    CHECK(hipMemcpyAsync(h.data(), D->wss, sizeof(double) * D->wall_nodes.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CHECK(hipStreamSynchronize(ctx->stream));
+    std::memset(wss, 0, sizeof(double) * D->nsites);
+    for (size_t i = 0; i < D->wall_nodes.size(); ++i) wss[D->node_site[D->wall_nodes[i]]] = h[i];
+    return LBM_OK;
+}
+
+extern "C" int lbm_reduce(lbm_domain* D, double* mass, double* flux) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    CHECK(hipSetDevice(ctx->device));
+    int rc = ensure_pre(D);
+    if (rc) return rc;
+    CHECK(launch_reduce(D->rho, D->u, D->n, D->d_bcnode, D->d_bcface, (int64_t)D->bc_nodes.size(), D->d_faces,
+                        (int)D->faces.size(), D->red, ctx->stream));
+    std::vector<double> h(D->faces.size() + 2);
+    CHECK(hipMemcpyAsync(h.data(), D->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CHECK(hipStreamSynchronize(ctx->stream));
+    if (mass) *mass = h[0];
+    if (flux)
+        for (size_t b = 0; b < D->faces.size(); ++b) flux[b] = h[1 + b];
+    if (h[D->faces.size() + 1] > 0) return lbm_ctx_fail(ctx, LBM_ENAN, "non-finite density in the state");
+    return LBM_OK;
+}

@@ -1,0 +1,100 @@
+// FP64 kernels: the reference-exact step, wall shear stress and the FP64
+// conservation reductions.  Built with -ffp-contract=off: every expression
+// rounds as written, which is what makes the FP64 path bit-identical to the
+// reference arithmetic.
+#include "step_kernels.h"
+
+using namespace lbm;
+
+hipError_t launch_step_f64(int mode, const StepArgs& a, int block, hipStream_t s) {
+    if (a.q == 19) return launch_step_t<19, PrecF64>(mode, a, block, s);
+    return launch_step_t<27, PrecF64>(mode, a, block, s);
+}
+
+hipError_t launch_init_f64(int q, const InitArgs& a, int block, hipStream_t s) {
+    (void)block;
+    if (q == 19) return launch_init_t<19, PrecF64>(a, s);
+    return launch_init_t<27, PrecF64>(a, s);
+}
+
+// Simulation.cpp:1374-1421: mean over present neighbours x + c_d (the rest
+// link makes the node its own neighbour) of u_n - u_x, times mu, then |.|.
+template <int Q>
+__global__ __launch_bounds__(256) void k_wss(WssArgs a) {
+    using L = Lat<Q>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nwall) return;
+    const int32_t x = a.wall[i];
+    const int32_t p = a.pnode[x];
+    const int64_t N = a.n;
+    const double ux = a.u[x], uy = a.u[N + x], uz = a.u[2 * N + x];
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    int cnt = 0;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        const int32_t nb = d == L::rest ? x : a.map[p + a.off[d]];
+        if (nb < 0) continue;
+        g0 += (a.u[nb] - ux);
+        g1 += (a.u[N + nb] - uy);
+        g2 += (a.u[2 * N + nb] - uz);
+        cnt++;
+    }
+    if (cnt > 0) {
+        g0 /= cnt;
+        g1 /= cnt;
+        g2 /= cnt;
+    }
+    const double s0 = a.mu * g0, s1 = a.mu * g1, s2 = a.mu * g2;
+    a.wss[i] = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+}
+
+hipError_t launch_wss(int q, const WssArgs& a, hipStream_t s) {
+    if (a.nwall == 0) return hipSuccess;
+    if (!a.wall || !a.pnode || !a.map || !a.u || !a.wss) return hipErrorInvalidValue;
+    const int grid = (int)((a.nwall + 255) / 256);
+    if (q == 19) hipLaunchKernelGGL(k_wss<19>, dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_wss<27>, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// FP64 health check: out[0] = sum rho, out[1 + b] = sum rho u_axis over the
+// nodes of boundary condition b, out[1 + nbc] = number of non-finite rho.
+__global__ __launch_bounds__(256) void k_reduce(const double* rho, const double* u, int64_t n,
+                                                const int32_t* bnode, const int32_t* bface,
+                                                int64_t nbnd, const FaceTab* faces, int nbc,
+                                                double* out) {
+    __shared__ double part[256];
+    double m = 0.0, bad = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double r = rho[i];
+        if (isfinite(r)) m += r;
+        else bad += 1.0;
+    }
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&out[0], part[0]);
+    if (bad > 0.0) atomicAdd(&out[1 + nbc], bad);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbnd;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = bnode[i];
+        const int b = bface[i];
+        const int ax = faces[b].axis;
+        atomicAdd(&out[1 + b], rho[x] * u[(int64_t)ax * n + x]);
+    }
+}
+
+hipError_t launch_reduce(const double* rho, const double* u, int64_t n, const int32_t* bnode,
+                         const int32_t* bface, int64_t nbnd, const FaceTab* faces, int nbc,
+                         double* out, hipStream_t s) {
+    if (!rho || !u || !out || (nbnd > 0 && (!bnode || !bface || !faces))) return hipErrorInvalidValue;
+    hipError_t e = hipMemsetAsync(out, 0, sizeof(double) * (2 + nbc), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce, dim3(512), dim3(256), 0, s, rho, u, n, bnode, bface, nbnd, faces, nbc,
+                       out);
+    return hipGetLastError();
+}

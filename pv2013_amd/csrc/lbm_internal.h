@@ -1,0 +1,99 @@
+// Internal interfaces between the host domain builder (domain.cpp) and the
+// kernel translation units (kernels_f64.hip, kernels_lp.hip, voxelize.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "lbm_capi.h"
+
+constexpr int LBM_MAXBC = 32;
+constexpr int LBM_MAXWAVE = 16;
+constexpr int LBM_MAXK = 16;
+
+// One Zou-He boundary condition as the kernels see it.  The population index
+// sets of each face are resolved at compile time from the lattice tables.
+struct FaceTab {
+    int32_t kind, axis, sigma, waveform;
+    int32_t typo;  // x-face pressure outlet N_xz typo (Simulation.cpp:1124-1129)
+    int32_t pad_;
+    double rho;
+    double u[3];
+};
+
+struct WaveTab {
+    int32_t kind, nk;
+    double period, dt;
+    double a[LBM_MAXK], phi[LBM_MAXK];
+};
+
+// Everything a step / pre-state kernel reads.  Node arrays are indexed by the
+// packed node id n; boundary arrays by the boundary rank r of n.
+struct StepArgs {
+    int64_t n;              // fluid nodes
+    int64_t nrec;           // boundary records
+    int32_t nblocks;        // logical blocks (grid is rounded up to a multiple of 8)
+    int32_t q;
+    const int32_t* pnode;   // padded map index of node n
+    const int32_t* map;     // padded site -> node id or -1
+    const uint64_t* bbits;  // bit n: node n has a boundary record
+    const int32_t* bpre;    // records before word n/64
+    const uint8_t* btype;   // per record: flag value (1 fluid, 2 wall, 3+b)
+    const uint32_t* bbb;    // per record: populations taken by bounce-back
+    const double* bprof;    // per record: velocity profile factor
+    const void* bstale;     // [e][r] stale populations (double, or float deviations)
+    const FaceTab* faces;
+    const WaveTab* waves;
+    int64_t* iter;          // device iteration counter
+    uint32_t* ticket;       // blocks finished in the current launch
+    const void* gin;        // [d][n] post-collision populations
+    void* gout;
+    double omega;
+    int32_t off[27];        // padded map offset of c_d
+    // pre-state outputs (mode PRE): f' [d][n] FP64, rho [n], u [c][n]
+    double* fpre;
+    double* rho;
+    double* u;
+};
+
+struct InitArgs {
+    int64_t n;
+    int32_t nblocks;
+    const double* fpre;  // [d][n]
+    void* gout;
+    double omega;
+};
+
+struct WssArgs {
+    int64_t nwall;
+    int64_t n;
+    const int32_t* wall;   // wall node ids
+    const int32_t* pnode;
+    const int32_t* map;
+    const double* u;       // [c][n]
+    double mu;
+    int32_t off[27];
+    double* wss;           // per wall node
+};
+
+enum { LBM_MODE_STEP = 0, LBM_MODE_PRE = 1 };
+
+// kernels_f64.hip
+hipError_t launch_step_f64(int mode, const StepArgs& a, int block, hipStream_t s);
+hipError_t launch_init_f64(int q, const InitArgs& a, int block, hipStream_t s);
+hipError_t launch_wss(int q, const WssArgs& a, hipStream_t s);
+hipError_t launch_reduce(const double* rho, const double* u, int64_t n, const int32_t* bnode,
+                         const int32_t* bface, int64_t nbnd, const FaceTab* faces, int nbc,
+                         double* out /* 1 + nbc + 1 */, hipStream_t s);
+// kernels_lp.hip (storage LBM_STORE_F32 or LBM_STORE_F16S)
+hipError_t launch_step_lp(int storage, int mode, const StepArgs& a, int block, hipStream_t s);
+hipError_t launch_init_lp(int storage, int q, const InitArgs& a, int block, hipStream_t s);
+
+// Grid rounded up to whole groups of 8 blocks so the XCD-aware block remap is a
+// bijection (blocks are dispatched round-robin over the 8 XCDs).
+inline int lbm_grid_blocks(int64_t items, int block) {
+    int64_t b = (items + block - 1) / block;
+    return (int)(((b + 7) / 8) * 8);
+}

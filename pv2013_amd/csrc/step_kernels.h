@@ -1,0 +1,512 @@
+// Fused stream-collide kernels for gfx950 (templated on lattice, precision and
+// mode).  Included by kernels_f64.hip (built with -ffp-contract=off so the
+// FP64 path rounds exactly like the reference) and kernels_lp.hip (FP32 /
+// FP16 shifted storage, FP32 collision).
+//
+// One thread per fluid node, node ids packed in an order whose fastest axis is
+// the vessel direction, populations SoA [d][n].  Per step and node:
+//   pull   f[d] = g_in[d][node at x - c_d]      (Simulation.cpp:184-198 as a pull)
+//   links  missing source: bounce-back g_in[opp d][x] or the stale value
+//          (Simulation.cpp:1353-1370; the reference leaves unlinked
+//          populations untouched)
+//   BC     Zou-He velocity / pressure on inlet / outlet nodes
+//          (Boundary.cpp:127-148, Simulation.cpp:952-1348)
+//   BGK    rho, u, f_eq, g_out = omega f_eq + (1 - omega) f (Simulation.cpp:21-179)
+// The stored g is the post-collision state; the reference's state f after the
+// last step is recovered by the same pull + BC without the collision (mode PRE).
+#pragma once
+
+#include "lattice.h"
+#include "lbm_internal.h"
+
+namespace lbm {
+
+struct PrecF64 {
+    using store_t = double;
+    using calc_t = double;
+};
+struct PrecF32S {
+    using store_t = float;
+    using calc_t = float;
+};
+struct PrecF16S {
+    using store_t = _Float16;
+    using calc_t = float;
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical blocks so neighbouring rows share an L2.
+__device__ __forceinline__ int64_t logical_block() {
+    const int per = gridDim.x >> 3;
+    return (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// Advance the device iteration counter once every block of the launch is done.
+__device__ __forceinline__ void finish_launch(int64_t* iter, uint32_t* ticket) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            *iter = *iter + 1;
+            *ticket = 0u;
+            __threadfence();
+        }
+    }
+}
+
+// ------------------------------------------------------------- waveforms (FP64)
+
+// pulsatile.h:12-28
+__device__ inline double pv_eq1(double x) {
+    return (-8.579e+08 * pow(x, 8) + 4.059e+08 * pow(x, 7) - 7.817e+07 * pow(x, 6) +
+            7.994e+06 * pow(x, 5) - 4.835e+05 * pow(x, 4) + 1.782e+04 * pow(x, 3) -
+            321.6 * pow(x, 2) + 3.215 * x + 0.1423);
+}
+__device__ inline double pv_eq2(double x) {
+    return (-4.32e+06 * pow(x, 8) + 7.94e+06 * pow(x, 7) - 6.298e+06 * pow(x, 6) +
+            2.82e+06 * pow(x, 5) - 7.809e+05 * pow(x, 4) + 1.37e+05 * pow(x, 3) -
+            1.484e+04 * pow(x, 2) + 901.9 * x - 22.87);
+}
+__device__ inline double pv_eq3(double x) {
+    return (-5.123 * pow(x, 4) + 14.15 * pow(x, 3) - 14.21 * pow(x, 2) + 5.865 * x - 0.5696);
+}
+
+// Waveform factor at iteration `it` (Simulation.cpp:875-900 for the PV2013
+// polynomial: the first evaluation, at t = 0, skips the period reduction).
+__device__ inline double wave_value(const WaveTab& w, int64_t it) {
+    if (w.kind == LBM_WAVE_FOURIER) {
+        double acc = 0.0;
+        for (int k = 0; k < w.nk; ++k) {
+            const double arg = 6.283185307179586 * (double)(k + 1) * (double)it / w.period + w.phi[k];
+            acc = acc + w.a[k] * sin(arg);
+        }
+        return 1.0 + acc;
+    }
+    if (w.kind == LBM_WAVE_PV2013) {
+        double x = (double)it * w.dt;
+        const double period = w.period;
+        if (it != 0) x -= period * floor(x / period);
+        const double nx = fmod(x, period);
+        if (nx >= 0 && nx < (0.1 * period)) return pv_eq1(nx);
+        if (nx >= (0.1 * period) && nx < (0.3 * period)) return pv_eq2(nx);
+        if (nx >= (0.3 * period) && nx < (0.9 * period)) return pv_eq3(nx);
+        return 0.0;
+    }
+    return 1.0;
+}
+
+// ------------------------------------------------------------ FP64 (reference)
+
+// Simulation.cpp:32-78 / :82-152: rho summed in index order; u from the +/-
+// groups in ascending index order, divided by rho.
+template <int Q>
+__device__ __forceinline__ void moments_f64(const double* f, double& rho, double* u) {
+    using L = Lat<Q>;
+    double r = f[0];
+#pragma unroll
+    for (int d = 1; d < Q; ++d) r = r + f[d];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double p = 0.0, m = 0.0;
+        bool fp = true, fm = true;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            if (L::c(d, a) == 1) { p = fp ? f[d] : p + f[d]; fp = false; }
+            if (L::c(d, a) == -1) { m = fm ? f[d] : m + f[d]; fm = false; }
+        }
+        u[a] = (p - m) / r;
+    }
+    rho = r;
+}
+
+// Simulation.cpp:154-176, written as the reference writes it.
+template <int Q>
+__device__ __forceinline__ void collide_f64(const double* f, double omega, double* g) {
+    using L = Lat<Q>;
+    double rho, u[3];
+    moments_f64<Q>(f, rho, u);
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        const double du = u[0] * (double)L::c(d, 0) + u[1] * (double)L::c(d, 1) + u[2] * (double)L::c(d, 2);
+        const double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double feq = L::w(d) * rho * (1.0 + 3.0 * du + 9.0 / 2.0 * du * du - 3.0 / 2.0 * usq);
+        g[d] = omega * feq + (1.0 - omega) * f[d];
+    }
+}
+
+// Zou-He on the face whose unknown populations have c[AX] == SG, in the
+// rounding order of Boundary.cpp:131-147 (velocity) and Simulation.cpp:966-1346
+// (pressure); D3Q27 follows Boundary.cpp:101-121.  All index sets are resolved
+// at compile time.
+template <int Q, int AX, int SG>
+__device__ void zouhe_f64(const FaceTab& F, const double* uv, bool typo, double* f) {
+    using L = Lat<Q>;
+    constexpr int T0 = AX == 0 ? 1 : 0;
+    constexpr int T1 = AX == 2 ? 1 : 2;
+    double S0 = 0.0, S1 = 0.0;
+    bool f0 = true, f1 = true;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        if (L::c(d, AX) == 0) { S0 = f0 ? f[d] : S0 + f[d]; f0 = false; }
+        if (L::c(d, AX) == -SG) { S1 = f1 ? f[d] : S1 + f[d]; f1 = false; }
+    }
+    double rho, u[3];
+    if (F.kind == LBM_BC_VELOCITY) {
+        u[0] = uv[0]; u[1] = uv[1]; u[2] = uv[2];
+        const double un = SG > 0 ? u[AX] : -u[AX];
+        rho = (S0 + 2.0 * S1) / (1 - un);
+    } else {
+        rho = F.rho;
+        const double v = 1 - (1.0 / rho) * (S0 + 2.0 * S1);
+        u[0] = 0.0; u[1] = 0.0; u[2] = 0.0;
+        u[AX] = SG > 0 ? v : -v;
+    }
+    double N[3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int t = k == 0 ? T0 : T1;
+        double P = 0.0, M = 0.0;
+        bool fp = true, fm = true;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            if (L::c(d, AX) == 0 && L::c(d, t) == 1) { P = fp ? f[d] : P + f[d]; fp = false; }
+            if (L::c(d, AX) == 0 && L::c(d, t) == -1) {
+                if (Q == 19) { M = fm ? f[d] : M + f[d]; fm = false; }
+            }
+        }
+        if (Q == 19) {
+            if (AX == 0 && SG == 1 && t == 2 && typo && F.kind == LBM_BC_PRESSURE)
+                P = f[4] + f[10] + f[14];  // Simulation.cpp:1124-1129
+            N[t] = 0.5 * (P - M) - (1.0 / 3.0) * rho * u[t];
+        } else {
+            double s = P;  // Boundary.cpp:101-105: ((p1 + p2 + p3) - m1 - m2 - m3)
+#pragma unroll
+            for (int d = 0; d < Q; ++d)
+                if (L::c(d, AX) == 0 && L::c(d, t) == -1) s = s - f[d];
+            N[t] = (1.0 / 6) * s - (1.0 / 9) * (rho * u[t]);
+        }
+    }
+    double g[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) g[d] = f[d];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        if (L::c(d, AX) != SG) continue;
+        const int e = L::opp(d);
+        double cu = SG > 0 ? u[AX] : -u[AX];
+        int nt = 0;
+        if (L::c(d, T0) == 1) { cu = cu + u[T0]; ++nt; }
+        if (L::c(d, T0) == -1) { cu = cu - u[T0]; ++nt; }
+        if (L::c(d, T1) == 1) { cu = cu + u[T1]; ++nt; }
+        if (L::c(d, T1) == -1) { cu = cu - u[T1]; ++nt; }
+        double coef;
+        if (Q == 19) coef = nt == 0 ? 1.0 / 3.0 : 1.0 / 6.0;
+        else coef = nt == 0 ? 4 / 9.0 : (nt == 1 ? 1 / 9.0 : 1 / 36.0);
+        double v = g[e] + coef * rho * cu;
+        if (L::c(d, T0) == 1) v = v - N[T0];
+        if (L::c(d, T0) == -1) v = v + N[T0];
+        if (L::c(d, T1) == 1) v = v - N[T1];
+        if (L::c(d, T1) == -1) v = v + N[T1];
+        f[d] = v;
+    }
+}
+
+// ----------------------------------------------------- FP32 on shifted values
+
+// Moments of the deviations f - w: rho = 1 + sum dev, j = sum c dev.
+template <int Q>
+__device__ __forceinline__ void moments_dev(const float* fd, float& drho, float* u) {
+    using L = Lat<Q>;
+    float r = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        r += fd[d];
+        if (L::c(d, 0)) j0 += L::c(d, 0) > 0 ? fd[d] : -fd[d];
+        if (L::c(d, 1)) j1 += L::c(d, 1) > 0 ? fd[d] : -fd[d];
+        if (L::c(d, 2)) j2 += L::c(d, 2) > 0 ? fd[d] : -fd[d];
+    }
+    const float inv = 1.f / (1.f + r);
+    drho = r;
+    u[0] = j0 * inv; u[1] = j1 * inv; u[2] = j2 * inv;
+}
+
+// BGK on deviations: g - w = (f - w) + omega ((f_eq - w) - (f - w)),
+// f_eq - w = w (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).
+template <int Q>
+__device__ __forceinline__ void collide_dev(float* fd, float omega) {
+    using L = Lat<Q>;
+    float drho, u[3];
+    moments_dev<Q>(fd, drho, u);
+    const float rho = 1.f + drho;
+    const float usq15 = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        float cu = 0.f;
+        if (L::c(d, 0)) cu += L::c(d, 0) > 0 ? u[0] : -u[0];
+        if (L::c(d, 1)) cu += L::c(d, 1) > 0 ? u[1] : -u[1];
+        if (L::c(d, 2)) cu += L::c(d, 2) > 0 ? u[2] : -u[2];
+        const float feqd = (float)L::w(d) * (drho + rho * (cu * (3.f + 4.5f * cu) - usq15));
+        fd[d] = fd[d] + omega * (feqd - fd[d]);
+    }
+}
+
+// Zou-He on deviations.  The weight parts cancel: the c_axis = 0 set plus twice
+// the outgoing set weighs 1, the +/- tangential sets weigh the same, and w_e =
+// w_opp(e), so the closure is the FP64 one with sum f -> 1 + sum dev.
+template <int Q, int AX, int SG>
+__device__ void zouhe_dev(const FaceTab& F, const float* uv, bool typo, float* fd) {
+    using L = Lat<Q>;
+    constexpr int T0 = AX == 0 ? 1 : 0;
+    constexpr int T1 = AX == 2 ? 1 : 2;
+    float D = 0.f;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        if (L::c(d, AX) == 0) D += fd[d];
+        if (L::c(d, AX) == -SG) D += 2.f * fd[d];
+    }
+    float rho, u[3];
+    if (F.kind == LBM_BC_VELOCITY) {
+        u[0] = uv[0]; u[1] = uv[1]; u[2] = uv[2];
+        const float un = SG > 0 ? u[AX] : -u[AX];
+        rho = (1.f + D) / (1.f - un);
+    } else {
+        rho = (float)F.rho;
+        const float v = ((float)(F.rho - 1.0) - D) / rho;
+        u[0] = 0.f; u[1] = 0.f; u[2] = 0.f;
+        u[AX] = SG > 0 ? v : -v;
+    }
+    float N[3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int t = k == 0 ? T0 : T1;
+        float PM = 0.f;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            if (L::c(d, AX) == 0 && L::c(d, t) == 1) PM += fd[d];
+            if (L::c(d, AX) == 0 && L::c(d, t) == -1) PM -= fd[d];
+        }
+        if (Q == 19) {
+            if (AX == 0 && SG == 1 && t == 2 && typo && F.kind == LBM_BC_PRESSURE)
+                PM += fd[10] - fd[16];  // Simulation.cpp:1124-1129
+            N[t] = 0.5f * PM - (1.f / 3.f) * rho * u[t];
+        } else {
+            N[t] = (1.f / 6.f) * PM - (1.f / 9.f) * rho * u[t];
+        }
+    }
+    float g[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) g[d] = fd[d];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        if (L::c(d, AX) != SG) continue;
+        float cu = SG > 0 ? u[AX] : -u[AX];
+        int nt = 0;
+        if (L::c(d, T0)) { cu += L::c(d, T0) > 0 ? u[T0] : -u[T0]; ++nt; }
+        if (L::c(d, T1)) { cu += L::c(d, T1) > 0 ? u[T1] : -u[T1]; ++nt; }
+        float coef;
+        if (Q == 19) coef = nt == 0 ? 1.f / 3.f : 1.f / 6.f;
+        else coef = nt == 0 ? 4.f / 9.f : (nt == 1 ? 1.f / 9.f : 1.f / 36.f);
+        float v = g[L::opp(d)] + coef * rho * cu;
+        if (L::c(d, T0)) v += L::c(d, T0) > 0 ? -N[T0] : N[T0];
+        if (L::c(d, T1)) v += L::c(d, T1) > 0 ? -N[T1] : N[T1];
+        fd[d] = v;
+    }
+}
+
+// ---------------------------------------------------------------- dispatch
+
+template <int Q, class T>
+struct ZouHe;
+
+template <int Q>
+struct ZouHe<Q, double> {
+    __device__ static void apply(const FaceTab& F, const double* uv, double* f) {
+        const bool typo = F.typo != 0;
+        switch (F.axis * 2 + (F.sigma > 0)) {
+            case 0: zouhe_f64<Q, 0, -1>(F, uv, typo, f); break;
+            case 1: zouhe_f64<Q, 0, 1>(F, uv, typo, f); break;
+            case 2: zouhe_f64<Q, 1, -1>(F, uv, typo, f); break;
+            case 3: zouhe_f64<Q, 1, 1>(F, uv, typo, f); break;
+            case 4: zouhe_f64<Q, 2, -1>(F, uv, typo, f); break;
+            default: zouhe_f64<Q, 2, 1>(F, uv, typo, f); break;
+        }
+    }
+};
+
+template <int Q>
+struct ZouHe<Q, float> {
+    __device__ static void apply(const FaceTab& F, const double* uvd, float* f) {
+        const bool typo = F.typo != 0;
+        const float uv[3] = {(float)uvd[0], (float)uvd[1], (float)uvd[2]};
+        switch (F.axis * 2 + (F.sigma > 0)) {
+            case 0: zouhe_dev<Q, 0, -1>(F, uv, typo, f); break;
+            case 1: zouhe_dev<Q, 0, 1>(F, uv, typo, f); break;
+            case 2: zouhe_dev<Q, 1, -1>(F, uv, typo, f); break;
+            case 3: zouhe_dev<Q, 1, 1>(F, uv, typo, f); break;
+            case 4: zouhe_dev<Q, 2, -1>(F, uv, typo, f); break;
+            default: zouhe_dev<Q, 2, 1>(F, uv, typo, f); break;
+        }
+    }
+};
+
+template <class P>
+__device__ __forceinline__ typename P::calc_t load_pop(const typename P::store_t* g, int64_t i) {
+    return (typename P::calc_t)g[i];
+}
+
+// ------------------------------------------------------------------- kernel
+
+template <int Q, class P, int MODE>
+__global__ __launch_bounds__(256) void k_step(StepArgs a) {
+    using L = Lat<Q>;
+    using S = typename P::store_t;
+    using T = typename P::calc_t;
+    constexpr bool F64 = sizeof(T) == 8;
+    const int64_t lb = logical_block();
+    const int64_t n = lb * blockDim.x + threadIdx.x;
+    if (lb < a.nblocks && n < a.n) {
+        const S* __restrict__ gin = (const S*)a.gin;
+        const int64_t N = a.n;
+        const int32_t p = a.pnode[n];
+        T f[Q];
+        uint32_t miss = 0;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            if (d == L::rest) {
+                f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+            } else {
+                const int32_t src = a.map[p - a.off[d]];
+                if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)d * N + src);
+                else { f[d] = T(0); miss |= 1u << d; }
+            }
+        }
+        const uint64_t word = a.bbits[n >> 6];
+        const int bit = (int)(n & 63);
+        if ((word >> bit) & 1ull) {
+            const int64_t r = a.bpre[n >> 6] + __popcll(word & ((1ull << bit) - 1ull));
+            const uint32_t bb = a.bbb[r];
+            if (miss) {
+                const T* stale = (const T*)a.bstale;
+#pragma unroll
+                for (int d = 0; d < Q; ++d) {
+                    if (!((miss >> d) & 1u)) continue;
+                    if ((bb >> d) & 1u) f[d] = load_pop<P>(gin, (int64_t)L::opp(d) * N + n);
+                    else f[d] = stale[(int64_t)d * a.nrec + r];
+                }
+            }
+            const int type = a.btype[r];
+            if (type >= 3) {
+                const FaceTab& F = a.faces[type - 3];
+                double uv[3] = {0.0, 0.0, 0.0};
+                if (F.kind == LBM_BC_VELOCITY) {
+                    double w = 1.0;
+                    if (F.waveform >= 0) {
+                        const int64_t it = *a.iter - (MODE == LBM_MODE_PRE ? 1 : 0);
+                        w = wave_value(a.waves[F.waveform], it);
+                    }
+                    const double fac = a.bprof[r] * w;
+                    uv[0] = F.u[0] * fac; uv[1] = F.u[1] * fac; uv[2] = F.u[2] * fac;
+                }
+                ZouHe<Q, T>::apply(F, uv, f);
+            }
+        }
+        if (MODE == LBM_MODE_STEP) {
+            S* __restrict__ gout = (S*)a.gout;
+            if constexpr (F64) {
+                double g[Q];
+                collide_f64<Q>((const double*)f, a.omega, g);
+#pragma unroll
+                for (int d = 0; d < Q; ++d) gout[(int64_t)d * N + n] = (S)g[d];
+            } else {
+                collide_dev<Q>((float*)f, (float)a.omega);
+#pragma unroll
+                for (int d = 0; d < Q; ++d) gout[(int64_t)d * N + n] = (S)f[d];
+            }
+        } else {
+            double fd[Q];
+#pragma unroll
+            for (int d = 0; d < Q; ++d) fd[d] = F64 ? (double)f[d] : L::w(d) + (double)f[d];
+#pragma unroll
+            for (int d = 0; d < Q; ++d) a.fpre[(int64_t)d * N + n] = fd[d];
+            double rho, u[3];
+            if constexpr (F64) {
+                moments_f64<Q>(fd, rho, u);
+            } else {
+                double dr = 0.0, j[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int d = 0; d < Q; ++d) {
+                    const double v = (double)f[d];
+                    dr += v;
+                    j[0] += L::c(d, 0) * v; j[1] += L::c(d, 1) * v; j[2] += L::c(d, 2) * v;
+                }
+                rho = 1.0 + dr;
+                u[0] = j[0] / rho; u[1] = j[1] / rho; u[2] = j[2] / rho;
+            }
+            a.rho[n] = rho;
+            a.u[n] = u[0]; a.u[N + n] = u[1]; a.u[2 * N + n] = u[2];
+        }
+    }
+    if (MODE == LBM_MODE_STEP) finish_launch(a.iter, a.ticket);
+}
+
+template <int Q, class P>
+__global__ __launch_bounds__(256) void k_init(InitArgs a) {
+    using L = Lat<Q>;
+    using S = typename P::store_t;
+    using T = typename P::calc_t;
+    const int64_t lb = logical_block();
+    const int64_t n = lb * blockDim.x + threadIdx.x;
+    if (lb >= a.nblocks || n >= a.n) return;
+    S* gout = (S*)a.gout;
+    if constexpr (sizeof(T) == 8) {
+        double f[Q], g[Q];
+#pragma unroll
+        for (int d = 0; d < Q; ++d) f[d] = a.fpre ? a.fpre[(int64_t)d * a.n + n] : L::w(d);
+        collide_f64<Q>(f, a.omega, g);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) gout[(int64_t)d * a.n + n] = (S)g[d];
+    } else {
+        float f[Q];
+#pragma unroll
+        for (int d = 0; d < Q; ++d) f[d] = a.fpre ? (float)(a.fpre[(int64_t)d * a.n + n] - L::w(d)) : 0.f;
+        collide_dev<Q>(f, (float)a.omega);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) gout[(int64_t)d * a.n + n] = (S)f[d];
+    }
+}
+
+// Every pointer a launch dereferences must be set; a missing one is an error
+// returned to the caller, never a launch.
+inline bool step_args_ok(int mode, const StepArgs& a) {
+    if (!a.pnode || !a.map || !a.bbits || !a.bpre || !a.gin || !a.faces || !a.waves || !a.iter) return false;
+    if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale)) return false;
+    if (mode == LBM_MODE_STEP) return a.gout && a.ticket;
+    return a.fpre && a.rho && a.u;
+}
+
+template <int Q, class P>
+hipError_t launch_step_t(int mode, const StepArgs& a, int block, hipStream_t s) {
+    if (!step_args_ok(mode, a)) return hipErrorInvalidValue;
+    const int grid = lbm_grid_blocks(a.n, 256);
+    StepArgs b = a;
+    b.nblocks = (int)((a.n + 255) / 256);
+    (void)block;
+    if (mode == LBM_MODE_STEP)
+        hipLaunchKernelGGL((k_step<Q, P, LBM_MODE_STEP>), dim3(grid), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_step<Q, P, LBM_MODE_PRE>), dim3(grid), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+template <int Q, class P>
+hipError_t launch_init_t(const InitArgs& a, hipStream_t s) {
+    if (!a.gout) return hipErrorInvalidValue;
+    const int grid = lbm_grid_blocks(a.n, 256);
+    InitArgs b = a;
+    b.nblocks = (int)((a.n + 255) / 256);
+    hipLaunchKernelGGL((k_init<Q, P>), dim3(grid), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -1,0 +1,361 @@
+// STL voxelizer and boundary classification for gfx950.
+//
+// Geometry::labelPoints (ComplexGeometry.cpp:431-506) casts, for every grid
+// point, a +z ray of length (max_z - min_z) + 0.1 and counts the facets whose
+// plane (through v1, with the normal read from the file) it crosses with
+// t in [-1e-7, 1 + 1e-7] at a point inside the facet's XY projection
+// (inclusive sign test, ComplexGeometry.cpp:264-281).  The point is fluid iff
+// the count is odd.
+//
+// The XY test does not depend on z, so the work is organised per column:
+//   host   facets with nz == 0 are dropped (their t is +-inf or NaN, never in
+//          the window); facets whose XY projection is thinner than ~1e-6 rad
+//          go to a list every column tests; the rest are binned into 8x8
+//          column tiles by their XY bbox widened by a margin that bounds the
+//          rounding error of the sign test (so binning never loses a facet the
+//          exact test would accept);
+//   device one 64-lane workgroup per tile, one lane per column; facets staged
+//          through LDS 64 at a time; for each facet that passes the exact XY
+//          test the lane evaluates the reference's t expression for every k
+//          and toggles a parity bit held in LDS; bits are written out as flags.
+// Built with -ffp-contract=off so t and the sign test round as the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lbm_internal.h"
+
+int lbm_ctx_fail(lbm_ctx* ctx, int code, const std::string& msg);
+hipStream_t lbm_ctx_stream(lbm_ctx* ctx);
+int lbm_ctx_device(lbm_ctx* ctx);
+
+namespace {
+
+constexpr int TILE = 8;           // 8x8 columns per workgroup
+constexpr int MAXNZ = 2048;       // parity bits per column held in LDS
+constexpr int NZW = MAXNZ / 32;
+
+struct VoxArgs {
+    const double* tri;        // 12 doubles per facet
+    const int32_t* tile_off;  // ntiles + 1
+    const int32_t* tile_tri;
+    const int32_t* global_tri;
+    int32_t nglobal;
+    int32_t nx, ny, nz;
+    int32_t ntx;
+    double ox, oy, oz, sp;
+    double bminx, bmaxx, bminy, bmaxy;
+    double L, pad, off, teps;
+    uint8_t* flags;
+};
+
+// ComplexGeometry.cpp:264-281
+__device__ __forceinline__ bool inside_xy(double x, double y, const double* T) {
+    const double x1 = T[0], y1 = T[1], x2 = T[3], y2 = T[4], x3 = T[6], y3 = T[7];
+    const double d1 = ((x - x2) * (y1 - y2) - (y - y2) * (x1 - x2));
+    const double d2 = ((x - x3) * (y2 - y3) - (y - y3) * (x2 - x3));
+    const double d3 = ((x - x1) * (y3 - y1) - (y - y1) * (x3 - x1));
+    const bool neg = (d1 < 0) || (d2 < 0) || (d3 < 0);
+    const bool pos = (d1 > 0) || (d2 > 0) || (d3 > 0);
+    return !(neg && pos);
+}
+
+__device__ void process_list(const VoxArgs& a, const int32_t* list, int count, double* sT,
+                             uint32_t* bits, bool valid, double sx, double sy, double ex, double ey) {
+    const int tid = threadIdx.x;
+    for (int base = 0; base < count; base += 64) {
+        const int m = min(64, count - base);
+        __syncthreads();
+        if (tid < m) {
+            const double* T = a.tri + 12 * (int64_t)list[base + tid];
+#pragma unroll
+            for (int c = 0; c < 12; ++c) sT[c * 64 + tid] = T[c];
+        }
+        __syncthreads();
+        if (!valid) continue;
+        for (int t = 0; t < m; ++t) {
+            double T[12];
+#pragma unroll
+            for (int c = 0; c < 12; ++c) T[c] = sT[c * 64 + t];
+            if (!inside_xy(sx, sy, T)) continue;
+            const double nx = T[9], ny = T[10], nz = T[11];
+            for (int k = 0; k < a.nz; ++k) {
+                // ComplexGeometry.cpp:441-458 then :315-316, literally
+                const double sz = a.oz + k * a.sp;
+                const double ez = sz + a.L + a.pad;
+                const double tt = ((T[0] - sx) * nx + (T[1] - sy) * ny + (T[2] - sz) * nz) /
+                                  ((ex - sx) * nx + (ey - sy) * ny + (ez - sz) * nz);
+                if (tt >= -a.teps && tt <= 1 + a.teps) bits[(k >> 5) * 64 + tid] ^= 1u << (k & 31);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_voxelize(VoxArgs a) {
+    __shared__ double sT[12 * 64];
+    __shared__ uint32_t bits[NZW * 64];
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int tx = tile % a.ntx, ty = tile / a.ntx;
+    const int i = tx * TILE + (tid % TILE), j = ty * TILE + (tid / TILE);
+    const bool valid = i < a.nx && j < a.ny;
+    const int nw = (a.nz + 31) >> 5;
+    for (int w = 0; w < nw; ++w) bits[w * 64 + tid] = 0u;
+    // the ray of ComplexGeometry.cpp:441-458 with adjustLineCoordinates case 0
+    double sx = a.ox + i * a.sp, sy = a.oy + j * a.sp;
+    double ex = sx, ey = sy;
+    if (sx == a.bminx) { sx += a.off; ex += a.off; }
+    if (sx == a.bmaxx) { sx -= a.off; ex -= a.off; }
+    if (sy == a.bminy) { sy += a.off; ey += a.off; }
+    if (sy == a.bmaxy) { sy -= a.off; ey -= a.off; }
+    const int beg = a.tile_off[tile], end = a.tile_off[tile + 1];
+    process_list(a, a.tile_tri + beg, end - beg, sT, bits, valid, sx, sy, ex, ey);
+    process_list(a, a.global_tri, a.nglobal, sT, bits, valid, sx, sy, ex, ey);
+    __syncthreads();
+    if (!valid) return;
+    const int64_t nxy = (int64_t)a.nx * a.ny;
+    const int64_t col = i + (int64_t)a.nx * j;
+    for (int k = 0; k < a.nz; ++k)
+        a.flags[col + nxy * k] = (uint8_t)((bits[(k >> 5) * 64 + tid] >> (k & 31)) & 1u);
+}
+
+struct ClsArgs {
+    const uint8_t* in;
+    uint8_t* out;
+    int32_t nx, ny, nz;
+    int32_t nplanes;
+    const int32_t* box;  // per plane: lo[3], hi[3] index intervals
+};
+
+// ComplexGeometry.cpp:509-556 (boundary: fluid with an outside 26-neighbour or
+// on a grid face), :579-638 (plane membership, here as index intervals worked
+// out on the host in long double) and Subdomain.cpp:413-473 (priority).
+__global__ __launch_bounds__(256) void k_classify(ClsArgs a) {
+    const int64_t nxy = (int64_t)a.nx * a.ny;
+    const int64_t ns = nxy * a.nz;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const uint8_t v = a.in[s];
+    if (v != 1) { a.out[s] = 0; return; }
+    const int i = (int)(s % a.nx), j = (int)((s / a.nx) % a.ny), k = (int)(s / nxy);
+    bool bnd = (i == 0 || i == a.nx - 1 || j == 0 || j == a.ny - 1 || k == 0 || k == a.nz - 1);
+    for (int dz = -1; dz <= 1 && !bnd; ++dz)
+        for (int dy = -1; dy <= 1 && !bnd; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = i + dx, y = j + dy, z = k + dz;
+                if (x < 0 || x >= a.nx || y < 0 || y >= a.ny || z < 0 || z >= a.nz) continue;
+                if (a.in[x + a.nx * (y + (int64_t)a.ny * z)] == 0) { bnd = true; break; }
+            }
+    if (!bnd) { a.out[s] = 1; return; }
+    uint8_t f = 2;
+    for (int p = 0; p < a.nplanes; ++p) {
+        const int32_t* b = a.box + 6 * p;
+        if (i >= b[0] && i <= b[3] && j >= b[1] && j <= b[4] && k >= b[2] && k <= b[5]) {
+            f = (uint8_t)(3 + p);
+            break;
+        }
+    }
+    a.out[s] = f;
+}
+
+struct DevMem {
+    std::vector<void*> ptrs;
+    ~DevMem() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t alloc(T** p, size_t count) {
+        void* v = nullptr;
+        hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(v);
+        *p = (T*)v;
+        return e;
+    }
+};
+
+// Sliver test: a facet whose XY projection has its smallest angle below about
+// 1e-6 rad is tested at every column instead of being binned.
+bool thin_xy(const double* T) {
+    const double ax = T[3] - T[0], ay = T[4] - T[1], bx = T[6] - T[0], by = T[7] - T[1];
+    const double cx = T[6] - T[3], cy = T[7] - T[4];
+    const double area2 = std::fabs(ax * by - ay * bx);
+    const double l = std::max(ax * ax + ay * ay, std::max(bx * bx + by * by, cx * cx + cy * cy));
+    return !(area2 >= 1e-6 * l) || l == 0.0;
+}
+
+}  // namespace
+
+#define VCHECK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return lbm_ctx_fail(ctx, LBM_EIO, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const lbm_grid_desc* g,
+                            uint8_t* flags_out, double* kernel_ms) {
+    if (!ctx) return LBM_EINVAL;
+    if (!g || !flags_out || (ntri > 0 && !tri) || ntri < 0)
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_voxelize: null argument");
+    const int nx = g->dims[0], ny = g->dims[1], nz = g->dims[2];
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(g->spacing > 0))
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_voxelize: bad grid");
+    if (nz > MAXNZ) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: nz above 2048");
+    if (ntri > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: too many facets");
+    VCHECK(hipSetDevice(lbm_ctx_device(ctx)));
+    hipStream_t st = lbm_ctx_stream(ctx);
+    const double ox = g->origin[0], oy = g->origin[1], sp = g->spacing;
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a)
+        ext = std::max(ext, std::fabs(g->raybox[2 * a + 1] - g->raybox[2 * a]) + std::fabs(g->origin[a]) +
+                                g->dims[a] * g->spacing);
+    const double margin = 1e-7 * std::max(ext, 1.0);
+    const int ntx = (nx + TILE - 1) / TILE, nty = (ny + TILE - 1) / TILE;
+    const int64_t ntiles = (int64_t)ntx * nty;
+    // bin facets into column tiles
+    std::vector<int32_t> keep, global;
+    std::vector<int32_t> cnt(ntiles + 1, 0);
+    std::vector<std::array<int32_t, 4>> rng;
+    for (int64_t t = 0; t < ntri; ++t) {
+        const double* T = tri + 12 * t;
+        if (T[11] == 0.0) continue;  // t = +-inf or NaN: never counted
+        if (thin_xy(T)) { global.push_back((int32_t)t); continue; }
+        const double slack = margin + g->ray_offset;
+        const double xlo = std::min({T[0], T[3], T[6]}) - slack, xhi = std::max({T[0], T[3], T[6]}) + slack;
+        const double ylo = std::min({T[1], T[4], T[7]}) - slack, yhi = std::max({T[1], T[4], T[7]}) + slack;
+        long i0 = (long)std::floor((xlo - ox) / sp) - 1, i1 = (long)std::ceil((xhi - ox) / sp) + 1;
+        long j0 = (long)std::floor((ylo - oy) / sp) - 1, j1 = (long)std::ceil((yhi - oy) / sp) + 1;
+        i0 = std::max(i0, 0L); j0 = std::max(j0, 0L);
+        i1 = std::min(i1, (long)nx - 1); j1 = std::min(j1, (long)ny - 1);
+        if (i0 > i1 || j0 > j1) continue;
+        std::array<int32_t, 4> r = {(int32_t)(i0 / TILE), (int32_t)(i1 / TILE), (int32_t)(j0 / TILE),
+                                    (int32_t)(j1 / TILE)};
+        keep.push_back((int32_t)t);
+        rng.push_back(r);
+        for (int ty = r[2]; ty <= r[3]; ++ty)
+            for (int tx = r[0]; tx <= r[1]; ++tx) cnt[(int64_t)ty * ntx + tx + 1]++;
+    }
+    for (int64_t i = 0; i < ntiles; ++i) cnt[i + 1] += cnt[i];
+    if (cnt[ntiles] > INT32_MAX / 2) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: bin lists too long");
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1), lists(std::max<int64_t>(cnt[ntiles], 1));
+    for (size_t u = 0; u < keep.size(); ++u) {
+        const auto& r = rng[u];
+        for (int ty = r[2]; ty <= r[3]; ++ty)
+            for (int tx = r[0]; tx <= r[1]; ++tx) lists[fill[(int64_t)ty * ntx + tx]++] = keep[u];
+    }
+    DevMem m;
+    double* d_tri;
+    int32_t *d_off, *d_list, *d_glob;
+    uint8_t* d_flags;
+    const int64_t nsites = (int64_t)nx * ny * nz;
+    VCHECK(m.alloc(&d_tri, (size_t)std::max<int64_t>(ntri, 1) * 12));
+    VCHECK(m.alloc(&d_off, (size_t)ntiles + 1));
+    VCHECK(m.alloc(&d_list, lists.size()));
+    VCHECK(m.alloc(&d_glob, std::max<size_t>(global.size(), 1)));
+    VCHECK(m.alloc(&d_flags, (size_t)nsites));
+    if (ntri > 0) VCHECK(hipMemcpyAsync(d_tri, tri, sizeof(double) * 12 * ntri, hipMemcpyHostToDevice, st));
+    VCHECK(hipMemcpyAsync(d_off, cnt.data(), sizeof(int32_t) * (ntiles + 1), hipMemcpyHostToDevice, st));
+    VCHECK(hipMemcpyAsync(d_list, lists.data(), sizeof(int32_t) * lists.size(), hipMemcpyHostToDevice, st));
+    if (!global.empty())
+        VCHECK(hipMemcpyAsync(d_glob, global.data(), sizeof(int32_t) * global.size(), hipMemcpyHostToDevice, st));
+    VoxArgs a;
+    a.tri = d_tri; a.tile_off = d_off; a.tile_tri = d_list; a.global_tri = d_glob;
+    a.nglobal = (int32_t)global.size();
+    a.nx = nx; a.ny = ny; a.nz = nz; a.ntx = ntx;
+    a.ox = ox; a.oy = oy; a.oz = g->origin[2]; a.sp = sp;
+    a.bminx = g->raybox[0]; a.bmaxx = g->raybox[1]; a.bminy = g->raybox[2]; a.bmaxy = g->raybox[3];
+    a.L = g->raybox[5] - g->raybox[4];
+    a.pad = g->ray_pad; a.off = g->ray_offset; a.teps = g->t_eps;
+    a.flags = d_flags;
+    hipEvent_t e0, e1;
+    VCHECK(hipEventCreate(&e0));
+    VCHECK(hipEventCreate(&e1));
+    VCHECK(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_voxelize, dim3((unsigned)ntiles), dim3(64), 0, st, a);
+    hipError_t le = hipGetLastError();
+    VCHECK(hipEventRecord(e1, st));
+    VCHECK(le);
+    VCHECK(hipMemcpyAsync(flags_out, d_flags, (size_t)nsites, hipMemcpyDeviceToHost, st));
+    VCHECK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    VCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
+    return LBM_OK;
+}
+
+// Index interval of grid points along one axis that satisfy a plane's
+// conditions on that axis, evaluated in long double as
+// ComplexGeometry.cpp:579-633 does (coordinates are monotone in the index).
+static void axis_interval(const lbm_grid_desc* g, const lbm_plane_desc* p, int axis, int32_t* lo, int32_t* hi) {
+    int32_t first = -1, last = -2;
+    const long double v = (long double)p->value_hi + (long double)p->value_lo;
+    for (int i = 0; i < g->dims[axis]; ++i) {
+        const double c = g->origin[axis] + i * g->spacing;
+        bool ok = true;
+        if (p->axis == axis) ok = std::fabs((long double)c - v) <= (long double)p->tol;
+        for (int r = 0; r < p->nranges && ok; ++r)
+            if (p->range_axis[r] == axis)
+                ok = ((long double)p->range_lo[r] < (long double)c) && ((long double)c < (long double)p->range_hi[r]);
+        if (ok) {
+            if (first < 0) first = i;
+            last = i;
+        }
+    }
+    *lo = first < 0 ? 1 : first;
+    *hi = first < 0 ? 0 : last;
+}
+
+extern "C" int lbm_classify(lbm_ctx* ctx, const lbm_grid_desc* g, const lbm_plane_desc* planes, int nplanes,
+                            uint8_t* flags, double* kernel_ms) {
+    if (!ctx) return LBM_EINVAL;
+    if (!g || !flags || nplanes < 0 || (nplanes > 0 && !planes))
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_classify: null argument");
+    if (nplanes > 252) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_classify: more than 252 planes");
+    for (int p = 0; p < nplanes; ++p) {
+        if (planes[p].axis < 0 || planes[p].axis > 2 || planes[p].nranges < 0 || planes[p].nranges > 3)
+            return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_classify: bad plane");
+        for (int r = 0; r < planes[p].nranges; ++r)
+            if (planes[p].range_axis[r] < 0 || planes[p].range_axis[r] > 2)
+                return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_classify: bad plane range axis");
+    }
+    const int nx = g->dims[0], ny = g->dims[1], nz = g->dims[2];
+    if (nx <= 0 || ny <= 0 || nz <= 0) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_classify: bad grid");
+    VCHECK(hipSetDevice(lbm_ctx_device(ctx)));
+    hipStream_t st = lbm_ctx_stream(ctx);
+    std::vector<int32_t> box(6 * std::max(nplanes, 1));
+    for (int p = 0; p < nplanes; ++p)
+        for (int a = 0; a < 3; ++a) axis_interval(g, &planes[p], a, &box[6 * p + a], &box[6 * p + 3 + a]);
+    const int64_t ns = (int64_t)nx * ny * nz;
+    DevMem m;
+    uint8_t *d_in, *d_out;
+    int32_t* d_box;
+    VCHECK(m.alloc(&d_in, (size_t)ns));
+    VCHECK(m.alloc(&d_out, (size_t)ns));
+    VCHECK(m.alloc(&d_box, box.size()));
+    VCHECK(hipMemcpyAsync(d_in, flags, (size_t)ns, hipMemcpyHostToDevice, st));
+    VCHECK(hipMemcpyAsync(d_box, box.data(), sizeof(int32_t) * box.size(), hipMemcpyHostToDevice, st));
+    ClsArgs a{d_in, d_out, nx, ny, nz, nplanes, d_box};
+    hipEvent_t e0, e1;
+    VCHECK(hipEventCreate(&e0));
+    VCHECK(hipEventCreate(&e1));
+    VCHECK(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, a);
+    hipError_t le = hipGetLastError();
+    VCHECK(hipEventRecord(e1, st));
+    VCHECK(le);
+    VCHECK(hipMemcpyAsync(flags, d_out, (size_t)ns, hipMemcpyDeviceToHost, st));
+    VCHECK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    VCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
+    return LBM_OK;
+}

@@ -1,0 +1,191 @@
+"""Seeded synthetic vessel meshes for the benchmark cases.
+
+Surfaces are extracted from an implicit vessel (union of tubes, capped by two
+planes) with marching tetrahedra, so every mesh is closed and the +z ray
+parity of the voxelizer is well defined.  The extraction grid is offset by
+irrational fractions of a lattice cell so that no lattice column runs along a
+cap edge (the reference's inclusive edge test counts such edges twice).
+
+All coordinates are in lattice units (spacing 1, lattice point (0,0,0) at the
+origin).  Triangles are (n, 12) float64: v1, v2, v3, unit normal (outward).
+"""
+import math
+
+import numpy as np
+
+_OFFSET = np.array([0.1234567, 0.3456789, 0.0])
+
+# Kuhn decomposition of the unit cube into 6 tetrahedra along the 0-7 diagonal
+_CORNER = np.array([[(c >> a) & 1 for a in range(3)] for c in range(8)])
+_TETS = []
+for _p in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+    v = [0]
+    acc = 0
+    for a in _p:
+        acc |= 1 << a
+        v.append(acc)
+    _TETS.append(v)
+
+
+def _case_tables():
+    """For each of the 16 inside/outside patterns of a tetrahedron, the
+    triangles as triples of edges (i, j) between an inside and outside vertex."""
+    table = {}
+    for case in range(1, 15):
+        ins = [i for i in range(4) if (case >> i) & 1]
+        out = [i for i in range(4) if not (case >> i) & 1]
+        if len(ins) == 1:
+            table[case] = [[(ins[0], out[0]), (ins[0], out[1]), (ins[0], out[2])]]
+        elif len(ins) == 3:
+            table[case] = [[(ins[0], out[0]), (ins[1], out[0]), (ins[2], out[0])]]
+        else:
+            a, b = ins
+            c, d = out
+            table[case] = [[(a, c), (a, d), (b, d)], [(a, c), (b, d), (b, c)]]
+    return table
+
+
+_TABLE = _case_tables()
+
+
+def marching_tets(sdf, lo, hi, h, slab=8):
+    """Zero level set of sdf (negative inside) on [lo, hi] with cell size h."""
+    lo = np.asarray(lo, float) + _OFFSET * h
+    n = np.maximum(np.ceil((np.asarray(hi, float) - lo) / h).astype(int), 1)
+    xs = lo[0] + h * np.arange(n[0] + 1)
+    ys = lo[1] + h * np.arange(n[1] + 1)
+    out = []
+    for k0 in range(0, n[2], slab):
+        k1 = min(n[2], k0 + slab)
+        zs = lo[2] + h * np.arange(k0, k1 + 1)
+        X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+        S = sdf(X, Y, Z)
+        if not ((S < 0).any() and (S >= 0).any()):
+            continue
+        nx, ny, nz = S.shape[0] - 1, S.shape[1] - 1, S.shape[2] - 1
+        gi, gj, gk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+        gi, gj, gk = gi.ravel(), gj.ravel(), gk.ravel()
+        cv = S[:-1, :-1, :-1].ravel()
+        sel = np.zeros(cv.shape, bool)
+        cvals = []
+        for c in range(8):
+            dx, dy, dz = _CORNER[c]
+            vals = S[dx:dx + nx, dy:dy + ny, dz:dz + nz].ravel()
+            cvals.append(vals)
+            sel |= (vals < 0) != (cv < 0)
+        idx = np.nonzero(sel)[0]
+        if idx.size == 0:
+            continue
+        gi, gj, gk = gi[idx], gj[idx], gk[idx]
+        cvals = [v[idx] for v in cvals]
+        cpos = [np.stack([xs[gi + _CORNER[c][0]], ys[gj + _CORNER[c][1]], zs[gk + _CORNER[c][2]]], 1)
+                for c in range(8)]
+        for tet in _TETS:
+            v = np.stack([cvals[t] for t in tet], 1)
+            p = np.stack([cpos[t] for t in tet], 1)
+            case = ((v < 0) * (1 << np.arange(4))).sum(1)
+            for cs, tris in _TABLE.items():
+                m = case == cs
+                if not m.any():
+                    continue
+                vv, pp = v[m], p[m]
+                ins = (cs >> np.arange(4)) & 1
+                pin = pp[:, ins == 1].mean(1)
+                pout = pp[:, ins == 0].mean(1)
+                for tri in tris:
+                    verts = []
+                    for (i, j) in tri:
+                        t = vv[:, i] / (vv[:, i] - vv[:, j])
+                        verts.append(pp[:, i] + t[:, None] * (pp[:, j] - pp[:, i]))
+                    a, b, c = verts
+                    nrm = np.cross(b - a, c - a)
+                    flip = (nrm * (pout - pin)).sum(1) < 0
+                    b2 = np.where(flip[:, None], c, b)
+                    c2 = np.where(flip[:, None], b, c)
+                    out.append(np.concatenate([a, b2, c2], 1))
+    if not out:
+        return np.zeros((0, 12))
+    V = np.concatenate(out, 0)
+    nrm = np.cross(V[:, 3:6] - V[:, 0:3], V[:, 6:9] - V[:, 0:3])
+    ln = np.linalg.norm(nrm, axis=1)
+    keep = ln > 1e-12
+    V, nrm, ln = V[keep], nrm[keep], ln[keep]
+    return np.concatenate([V, nrm / ln[:, None]], 1)
+
+
+class WallModes:
+    """Radial wall perturbation: sum of seeded Fourier modes in (angle, axial
+    position) whose amplitudes add up to `amp` lattice units."""
+
+    def __init__(self, rng, nmodes=8, amp=1.5, length=512.0):
+        self.m = rng.integers(1, 7, nmodes)
+        self.q = 2 * np.pi * rng.integers(1, 5, nmodes) / length
+        self.phi = 2 * np.pi * rng.random(nmodes)
+        a = rng.random(nmodes) + 0.2
+        self.a = amp * a / a.sum()
+
+    def __call__(self, theta, s):
+        out = np.zeros(np.broadcast(theta, s).shape)
+        for k in range(len(self.a)):
+            out += self.a[k] * np.cos(self.m[k] * theta + self.q[k] * s + self.phi[k])
+        return out
+
+
+def _tube_dist(dx, dy, r0, modes, s, cosang=1.0):
+    rh = np.hypot(dx, dy)
+    theta = np.arctan2(dy, dx)
+    return rh * cosang - (r0 + (modes(theta, s) if modes is not None else 0.0))
+
+
+def y_bifurcation(seed, dims=(256, 256, 512), parent_r=40.0, daughter_r=30.0, angle_deg=25.0,
+                  split_z=256.0, amp=1.5, nmodes=8, h=3.1, spread=80.0):
+    """Parent tube along z that splits at split_z into two daughters leaving at
+    +-angle in the x-z plane.  The daughters bend back towards z so they stay
+    inside the box: their centreline offset is spread*(1 - exp(-tan(a) dz/spread)).
+    Capped at z = 0.5 and z = nz - 1.5.  Returns (triangles, info)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    cx, cy = (nx - 1) / 2.0, (ny - 1) / 2.0
+    ta = math.tan(math.radians(angle_deg))
+    mp = WallModes(rng, nmodes, amp, nz)
+    md = [WallModes(rng, nmodes, amp, nz), WallModes(rng, nmodes, amp, nz)]
+    z0, z1 = 0.5, nz - 1.5
+
+    def sdf(X, Y, Z):
+        d = _tube_dist(X - cx, Y - cy, parent_r, mp, Z)
+        d = np.maximum(d, Z - (split_z + parent_r))
+        for sgn, modes in ((1.0, md[0]), (-1.0, md[1])):
+            dz = np.maximum(Z - split_z, 0.0)
+            off = spread * (1.0 - np.exp(-ta * dz / spread))
+            slope = ta * np.exp(-ta * dz / spread)
+            cosang = 1.0 / np.sqrt(1.0 + slope * slope)
+            dd = _tube_dist(X - (cx + sgn * off), Y - cy, daughter_r, modes, Z, cosang)
+            dd = np.maximum(dd, (split_z - daughter_r) - Z)
+            d = np.minimum(d, dd)
+        return np.maximum(d, np.maximum(z0 - Z, Z - z1))
+
+    tri = marching_tets(sdf, (cx - 130, cy - 50, 0.0), (cx + 130, cy + 50, nz - 1.0), h)
+    info = {"dims": dims, "center": (cx, cy), "inlet_z": 1, "outlet_z": nz - 2,
+            "parent_r": parent_r, "daughter_offset": spread * (1 - math.exp(-ta * (z1 - split_z) / spread))}
+    return tri, info
+
+
+def fourier_waveform(seed, nk=4, amp=0.5):
+    """Seeded a_k, phi_k for u(t) = U (1 + sum a_k sin(2 pi k t / T + phi_k))."""
+    rng = np.random.default_rng(seed + 7919)
+    a = rng.random(nk)
+    a = amp * a / a.sum() if a.sum() > 0 else a
+    phi = 2 * np.pi * rng.random(nk)
+    return a, phi
+
+
+def write_stl_ascii(path, tri, name="vessel"):
+    with open(path, "w") as fh:
+        fh.write("solid %s\n" % name)
+        lines = []
+        for t in tri:
+            lines.append("facet normal %.9e %.9e %.9e\n outer loop\n"
+                         "  vertex %.9e %.9e %.9e\n  vertex %.9e %.9e %.9e\n  vertex %.9e %.9e %.9e\n"
+                         " endloop\nendfacet\n" % (t[9], t[10], t[11], *t[:9]))
+        fh.write("".join(lines))
+        fh.write("endsolid %s\n" % name)

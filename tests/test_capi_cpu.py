@@ -1,0 +1,93 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol the
+header declares, its host-side STL readers and grid sizing match the
+reference's rules, and calls without a device fail with an error code."""
+import ctypes as C
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lbm_capi.h")
+
+
+@pytest.fixture(scope="module")
+def P():
+    import pv2013_amd
+    pv2013_amd.lib()
+    return pv2013_amd
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(P):
+    names = declared()
+    assert len(names) >= 20
+    L = P.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    from pv2013_amd import capi
+    assert set(capi.EXPORTS) <= set(names)
+
+
+def write_ascii_stl(path, tri):
+    with open(path, "w") as fh:
+        fh.write("solid t\n")
+        for t in tri:
+            fh.write("facet normal %.17g %.17g %.17g\n outer loop\n" % tuple(t[9:12]))
+            for v in range(3):
+                fh.write("  vertex %.17g %.17g %.17g\n" % tuple(t[3 * v:3 * v + 3]))
+            fh.write(" endloop\nendfacet\n")
+        fh.write("endsolid t\n")
+
+
+def test_stl_ascii_roundtrip_and_grid(P, tmp_path):
+    z = np.load(os.path.join(ROOT, "tests", "golden", "pv2013.npz"))
+    tri = z["tri"]
+    path = str(tmp_path / "pv.stl")
+    write_ascii_stl(path, tri)
+    got = P.read_stl(path)
+    assert np.array_equal(got, tri)
+    g = P.grid_from_bbox(got, 0.1)
+    assert list(g.dims) == [421, 351, 341]
+    assert g.origin[0] == tri[:, [0, 3, 6]].min() and g.raybox[5] == tri[:, [2, 5, 8]].max()
+
+
+def test_stl_binary_renormalises_in_float(P, tmp_path):
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal((5, 9)).astype(np.float32)
+    n = rng.standard_normal((5, 3)).astype(np.float32) * 3
+    path = str(tmp_path / "b.stl")
+    with open(path, "wb") as fh:
+        fh.write(b"\0" * 80 + struct.pack("<I", 5))
+        for i in range(5):
+            fh.write(n[i].tobytes() + v[i].tobytes() + b"\0\0")
+    got = P.read_stl(path)
+    norm = np.sqrt(n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1] + n[:, 2] * n[:, 2]).astype(np.float32)
+    assert np.array_equal(got[:, :9], v.astype(np.float64))
+    assert np.array_equal(got[:, 9:], (n / norm[:, None]).astype(np.float32).astype(np.float64))
+
+
+def test_calls_fail_cleanly_without_context(P):
+    L = P.lib()
+    h = C.c_void_p()
+    rc = L.lbm_ctx_create(0, C.byref(h))
+    if rc == 0:
+        L.lbm_ctx_destroy(h)
+        pytest.skip("a GPU is present")
+    assert rc == -19
+    assert L.lbm_domain_create(None, None, C.byref(h)) == -22
+    assert L.lbm_step(None, 1) == -22
+    assert L.lbm_get_f(None, None) == -22
+    assert L.lbm_last_error(None) == b"null context"
+
+
+def test_long_double_split_matches_oracle(P, oracle):
+    for text in ("-175.275", "-108.379", "0.1", "-37.0", "12345.678901"):
+        assert P.capi.split_long_double(text) == oracle.split_long_double(text)

@@ -131,7 +131,12 @@ typedef struct {
     const lbm_bc_desc* bcs;  /* bcs[b] drives flag 3+b */
     int32_t nbc;
     int32_t fast_axis;       /* node order: fastest-varying axis, -1 = longest runs */
+    int32_t options;         /* LBM_OPT_* bits */
 } lbm_domain_desc;
+
+/* lbm_domain_desc.options */
+#define LBM_OPT_NO_ROW_FASTPATH 1  /* address every lane through the site map */
+#define LBM_OPT_NO_GRAPH 2         /* launch every step eagerly instead of replaying a hipGraph */
 
 /* SubDomain(Geometry&) (Subdomain.cpp:3-33: CreateNodes, Connectivity,
  * Initialize, GetFluidHaloLat) + Simulation's constructor set-up
@@ -163,12 +168,34 @@ int lbm_get_f(lbm_domain* dom, double* f);
 int lbm_set_f(lbm_domain* dom, const double* f);
 /* Moments of the current state (Simulation.cpp:32-78): rho and u [c][site]. */
 int lbm_get_macros(lbm_domain* dom, double* rho, double* u);
+/* The same restricted to the box [origin, origin + dims) of the domain
+ * (arrays sized for that box, x fastest); used to read one case of a batch. */
+int lbm_get_macros_box(lbm_domain* dom, const int32_t origin[3], const int32_t dims[3],
+                       double* rho, double* u);
+int lbm_get_wss_box(lbm_domain* dom, const int32_t origin[3], const int32_t dims[3], double* wss);
 /* Wall shear stress of the current state on wall sites
  * (Simulation.cpp:1374-1421), zero elsewhere. */
 int lbm_get_wss(lbm_domain* dom, double* wss);
 /* FP64 health check: total mass and per-boundary-condition mass flux through
  * its face (flux has nbc entries, may be NULL).  LBM_ENAN on a non-finite state. */
 int lbm_reduce(lbm_domain* dom, double* mass, double* flux);
+
+/* ------------------------------------------------------------------- batch */
+
+typedef struct {
+    int32_t origin[3];  /* where the case's box starts inside the packed domain */
+    int32_t dims[3];
+    int32_t bc_base;    /* the case's boundary condition b is b + bc_base of the domain */
+    int32_t wave_base;  /* the case's waveform slot s is s + wave_base of the domain */
+} lbm_batch_case;
+
+/* Many independent cases of uneven size in one domain: boxes are stacked along
+ * `stack_axis` with one solid layer between them, so one launch per step (one
+ * graph replay per 64 steps) updates the whole ensemble.  All cases share q,
+ * storage, scheme, quirks and tau.  No reference counterpart (the reference
+ * runs one geometry per process). */
+int lbm_batch_create(lbm_ctx* ctx, const lbm_domain_desc* cases, int ncases, int stack_axis,
+                     lbm_domain** out, lbm_batch_case* info);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "csrc", "libpv2013_lbm.so")
 LBM_OK = 0
 STORE_F64, STORE_F32, STORE_F16S = 0, 1, 2
 SCHEME_TWO_LATTICE, SCHEME_AA = 0, 1
+OPT_NO_ROW_FASTPATH = 1
+OPT_NO_GRAPH = 2
 BC_VELOCITY, BC_PRESSURE = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 WAVE_NONE, WAVE_FOURIER, WAVE_PV2013 = 0, 1, 2
@@ -25,7 +27,7 @@ EXPORTS = (
     "lbm_read_stl_binary", "lbm_grid_from_bbox", "lbm_voxelize", "lbm_classify", "lbm_domain_create",
     "lbm_domain_destroy", "lbm_domain_info", "lbm_set_waveform", "lbm_step", "lbm_step_timed",
     "lbm_sync", "lbm_iteration", "lbm_get_f", "lbm_set_f", "lbm_get_macros", "lbm_get_wss",
-    "lbm_reduce",
+    "lbm_reduce", "lbm_get_macros_box", "lbm_get_wss_box", "lbm_batch_create",
 )
 
 
@@ -56,7 +58,12 @@ class DomainDesc(C.Structure):
     _fields_ = [("dims", C.c_int32 * 3), ("q", C.c_int32), ("storage", C.c_int32),
                 ("scheme", C.c_int32), ("quirks", C.c_int32), ("block", C.c_int32),
                 ("tau", C.c_double), ("flags", C.c_void_p), ("bcs", C.c_void_p),
-                ("nbc", C.c_int32), ("fast_axis", C.c_int32)]
+                ("nbc", C.c_int32), ("fast_axis", C.c_int32), ("options", C.c_int32)]
+
+
+class BatchCase(C.Structure):
+    _fields_ = [("origin", C.c_int32 * 3), ("dims", C.c_int32 * 3), ("bc_base", C.c_int32),
+                ("wave_base", C.c_int32)]
 
 
 _lib = None
@@ -97,6 +104,9 @@ def lib():
             getattr(L, fn).argtypes = [vp, vp]
         L.lbm_get_macros.argtypes = [vp, vp, vp]
         L.lbm_reduce.argtypes = [vp, P(dbl), vp]
+        L.lbm_get_macros_box.argtypes = [vp, vp, vp, vp, vp]
+        L.lbm_get_wss_box.argtypes = [vp, vp, vp, vp]
+        L.lbm_batch_create.argtypes = [vp, vp, i32, i32, P(vp), vp]
         _lib = L
     return _lib
 
@@ -215,31 +225,44 @@ class Context:
         return out, ms.value
 
 
+def domain_desc(dims, flags, tau, bcs=(), q=19, storage=STORE_F64, scheme=SCHEME_TWO_LATTICE,
+                quirks=1, block=0, fast_axis=-1, options=0):
+    """A DomainDesc plus the Python objects that must outlive it."""
+    dims = tuple(int(v) for v in dims)
+    flags = np.ascontiguousarray(flags, np.uint8).reshape(-1)
+    assert flags.size == dims[0] * dims[1] * dims[2]
+    arr = (BcDesc * max(1, len(bcs)))(*bcs)
+    d = DomainDesc()
+    d.dims[:] = list(dims)
+    d.q, d.storage, d.scheme, d.quirks, d.block = q, storage, scheme, quirks, block
+    d.tau = tau
+    d.flags = _ptr(flags)
+    d.bcs = C.cast(arr, C.c_void_p)
+    d.nbc = len(bcs)
+    d.fast_axis = fast_axis
+    d.options = options
+    return d, (flags, arr)
+
+
 class Domain:
     """One fluid domain on the GPU (SubDomain + Simulation of the reference)."""
 
     def __init__(self, ctx, dims, flags, tau, bcs=(), q=19, storage=STORE_F64,
-                 scheme=SCHEME_TWO_LATTICE, quirks=1, block=0, fast_axis=-1):
+                 scheme=SCHEME_TWO_LATTICE, quirks=1, block=0, fast_axis=-1, options=0, _handle=None):
         self.ctx = ctx
         self.q = q
         self.storage = storage
         self.dims = tuple(int(d) for d in dims)
         self.nsites = self.dims[0] * self.dims[1] * self.dims[2]
-        flags = np.ascontiguousarray(flags, np.uint8).reshape(-1)
-        assert flags.size == self.nsites
-        self._bcs = (BcDesc * max(1, len(bcs)))(*bcs)
-        d = DomainDesc()
-        d.dims[:] = list(self.dims)
-        d.q, d.storage, d.scheme, d.quirks, d.block = q, storage, scheme, quirks, block
-        d.tau = tau
-        d.flags = _ptr(flags)
-        d.bcs = C.cast(self._bcs, C.c_void_p)
-        d.nbc = len(bcs)
-        d.fast_axis = fast_axis
+        self.nbc = len(bcs)
+        if _handle is not None:
+            self.h = _handle
+            return
+        d, self._keep = domain_desc(dims, flags, tau, bcs, q, storage, scheme, quirks, block,
+                                    fast_axis, options)
         h = C.c_void_p()
         ctx.check(lib().lbm_domain_create(ctx.h, C.byref(d), C.byref(h)), "lbm_domain_create")
         self.h = h
-        self.nbc = len(bcs)
 
     def close(self):
         if getattr(self, "h", None):
@@ -296,8 +319,51 @@ class Domain:
         self.ctx.check(lib().lbm_get_wss(self.h, _ptr(out)), "lbm_get_wss")
         return out
 
+    def macros_box(self, origin, dims):
+        o = np.ascontiguousarray(origin, np.int32)
+        bd = np.ascontiguousarray(dims, np.int32)
+        nb = int(np.prod(bd))
+        rho = np.zeros(nb)
+        u = np.zeros(3 * nb)
+        self.ctx.check(lib().lbm_get_macros_box(self.h, _ptr(o), _ptr(bd), _ptr(rho), _ptr(u)),
+                       "lbm_get_macros_box")
+        return rho, u.reshape(3, nb)
+
+    def wss_box(self, origin, dims):
+        o = np.ascontiguousarray(origin, np.int32)
+        bd = np.ascontiguousarray(dims, np.int32)
+        out = np.zeros(int(np.prod(bd)))
+        self.ctx.check(lib().lbm_get_wss_box(self.h, _ptr(o), _ptr(bd), _ptr(out)), "lbm_get_wss_box")
+        return out
+
     def reduce(self):
         mass = C.c_double()
         flux = np.zeros(max(1, self.nbc))
         self.ctx.check(lib().lbm_reduce(self.h, C.byref(mass), _ptr(flux)), "lbm_reduce")
         return mass.value, flux[: self.nbc]
+
+
+def batch(ctx, cases, stack_axis=1):
+    """Pack several cases (each a dict of Domain arguments: dims, flags, tau,
+    bcs, q, storage, quirks, ...) into one domain.  Returns (Domain, info)
+    where info[c] = {"origin", "dims", "bc_base", "wave_base"}."""
+    descs, keep = [], []
+    for c in cases:
+        d, k = domain_desc(**c)
+        descs.append(d)
+        keep.append(k)
+    arr = (DomainDesc * len(descs))(*descs)
+    info = (BatchCase * len(descs))()
+    h = C.c_void_p()
+    ctx.check(lib().lbm_batch_create(ctx.h, C.cast(arr, C.c_void_p), len(descs), stack_axis, C.byref(h),
+                                     C.cast(info, C.c_void_p)), "lbm_batch_create")
+    out = [{"origin": tuple(i.origin), "dims": tuple(i.dims), "bc_base": i.bc_base,
+            "wave_base": i.wave_base} for i in info]
+    dims = [0, 0, 0]
+    for i in out:
+        for a in range(3):
+            dims[a] = max(dims[a], i["origin"][a] + i["dims"][a])
+    nbc = sum(len(c.get("bcs", ())) for c in cases)
+    dom = Domain(ctx, dims, None, cases[0]["tau"], [None] * nbc, q=cases[0].get("q", 19),
+                 storage=cases[0].get("storage", STORE_F64), _handle=h)
+    return dom, out

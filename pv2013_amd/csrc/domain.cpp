@@ -99,8 +99,11 @@ struct lbm_domain {
     // device arrays
     int32_t *pnode = nullptr, *map = nullptr, *bpre = nullptr;
     uint64_t* bbits = nullptr;
+    uint32_t* wreg = nullptr;  // null when the row fast path is off
+    int64_t nreg_waves = 0;
     uint8_t* btype = nullptr;
     uint32_t* bbb = nullptr;
+    uint32_t* bmiss = nullptr;
     double* bprof = nullptr;
     void* bstale = nullptr;
     FaceTab* d_faces = nullptr;
@@ -116,6 +119,11 @@ struct lbm_domain {
     int64_t pre_iter = -1;    // iteration for which fpre/rho/u are valid
     int32_t off[27];
     int block = 256;
+    int options = 0;
+    // hipGraph of GRAPH_STEPS consecutive steps, one per starting lattice
+    static constexpr int GRAPH_STEPS = 64;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
 
     template <class T>
     hipError_t alloc(T** p, size_t count) {
@@ -130,6 +138,10 @@ struct lbm_domain {
         return e;
     }
     ~lbm_domain() {
+        for (int c = 0; c < 2; ++c) {
+            if (exec[c]) (void)hipGraphExecDestroy(exec[c]);
+            if (graph[c]) (void)hipGraphDestroy(graph[c]);
+        }
         for (void* p : allocs) (void)hipFree(p);
     }
 };
@@ -216,9 +228,11 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.pnode = D->pnode;
     a.map = D->map;
     a.bbits = D->bbits;
+    a.wreg = D->wreg;
     a.bpre = D->bpre;
     a.btype = D->btype;
     a.bbb = D->bbb;
+    a.bmiss = D->bmiss;
     a.bprof = D->bprof;
     a.bstale = D->bstale;
     a.faces = D->d_faces;
@@ -229,7 +243,13 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     for (int d = 0; d < 27; ++d) a.off[d] = D->off[d];
 }
 
-static int launch_step(lbm_domain* D, int mode, const void* gin, void* gout) {
+// Streaming scheme of the step that starts at iteration `it`.
+static int scheme_for(const lbm_domain* D, int64_t it) {
+    if (D->scheme == LBM_SCHEME_TWO_LATTICE) return 0;
+    return (it & 1) == 0 ? 1 : 2;  // AA: gather on even iterations, local on odd
+}
+
+static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* gout) {
     lbm_ctx* ctx = D->ctx;
     StepArgs a;
     fill_args(D, a);
@@ -238,8 +258,8 @@ static int launch_step(lbm_domain* D, int mode, const void* gin, void* gout) {
     a.fpre = D->fpre;
     a.rho = D->rho;
     a.u = D->u;
-    hipError_t e = D->storage == LBM_STORE_F64 ? launch_step_f64(mode, a, D->block, ctx->stream)
-                                               : launch_step_lp(D->storage, mode, a, D->block, ctx->stream);
+    hipError_t e = D->storage == LBM_STORE_F64 ? launch_step_f64(mode, sch, a, ctx->stream)
+                                               : launch_step_lp(D->storage, mode, sch, a, ctx->stream);
     CHECK(e);
     return LBM_OK;
 }
@@ -252,8 +272,9 @@ static int init_from(lbm_domain* D, const double* fpre_dev) {
     a.fpre = fpre_dev;
     a.gout = D->g[D->cur];
     a.omega = 1.0 / D->tau;
-    hipError_t e = D->storage == LBM_STORE_F64 ? launch_init_f64(D->q, a, D->block, ctx->stream)
-                                               : launch_init_lp(D->storage, D->q, a, D->block, ctx->stream);
+    const bool swap = D->scheme == LBM_SCHEME_AA;
+    hipError_t e = D->storage == LBM_STORE_F64 ? launch_init_f64(D->q, swap, a, ctx->stream)
+                                               : launch_init_lp(D->storage, D->q, swap, a, ctx->stream);
     CHECK(e);
     return LBM_OK;
 }
@@ -273,7 +294,8 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     const lbm_domain_desc& d = *desc;
     if (d.q != 19 && d.q != 27) return lbm_ctx_fail(ctx, LBM_EINVAL, "q must be 19 or 27");
     if (d.storage < 0 || d.storage > 2) return lbm_ctx_fail(ctx, LBM_EINVAL, "bad storage");
-    if (d.scheme != LBM_SCHEME_TWO_LATTICE) return lbm_ctx_fail(ctx, LBM_EINVAL, "scheme not supported");
+    if (d.scheme != LBM_SCHEME_TWO_LATTICE && d.scheme != LBM_SCHEME_AA)
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "scheme must be two-lattice or AA");
     if (!(d.tau > 0.5)) return lbm_ctx_fail(ctx, LBM_EINVAL, "tau must exceed 0.5");
     for (int a = 0; a < 3; ++a)
         if (d.dims[a] <= 0) return lbm_ctx_fail(ctx, LBM_EINVAL, "dims must be positive");
@@ -296,6 +318,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     D->quirks = d.quirks;
     D->tau = d.tau;
     D->block = d.block > 0 ? d.block : 256;
+    D->options = d.options;
     for (int a = 0; a < 3; ++a) D->dims[a] = d.dims[a];
     const int nx = d.dims[0], ny = d.dims[1], nz = d.dims[2];
     D->nsites = (int64_t)nx * ny * nz;
@@ -355,7 +378,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     std::vector<uint64_t> bits((size_t)nwords, 0);
     std::vector<int32_t> pre((size_t)nwords, 0);
     std::vector<uint8_t> btype;
-    std::vector<uint32_t> bbb;
+    std::vector<uint32_t> bbb, bmiss;
     std::vector<double> bprof;
     std::vector<uint32_t> bstale_mask;
     D->faces.resize(std::max(d.nbc, 1));
@@ -392,6 +415,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
         }
         btype.push_back((uint8_t)type);
         bbb.push_back(bb);
+        bmiss.push_back(miss);
         bstale_mask.push_back(miss & ~bb);
         bprof.push_back(type >= 3 ? profile_factor(d.bcs[type - 3], ijk) : 1.0);
     }
@@ -400,6 +424,22 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     for (int64_t w = 0; w < nwords; ++w) {
         pre[w] = acc;
         acc += __builtin_popcountll(bits[w]);
+    }
+    // Wavefronts (64 consecutive node ids) made of interior nodes only, all in
+    // one row at consecutive positions along the fast axis: then for every
+    // direction the 64 sources are fluid, hence one contiguous run of ids.
+    const bool fastpath = !(d.options & LBM_OPT_NO_ROW_FASTPATH);
+    std::vector<uint32_t> wreg((size_t)((nwords + 31) / 32), 0u);
+    if (fastpath) {
+        for (int64_t w = 0; w < nwords; ++w) {
+            if ((w + 1) * 64 > D->n || bits[w] != 0) continue;
+            const int64_t p0 = pnode[w * 64], p1 = pnode[w * 64 + 63];
+            if (p1 - p0 != 63) continue;  // same row, consecutive positions
+            const int64_t row0 = p0 / D->pd[0], row1 = p1 / D->pd[0];
+            if (row0 != row1) continue;
+            wreg[w >> 5] |= 1u << (w & 31);
+            D->nreg_waves++;
+        }
     }
     // device allocation
     int rc = LBM_OK;
@@ -422,8 +462,10 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->map, (size_t)mapn));
     DCHECK(D->alloc(&D->bbits, (size_t)nwords));
     DCHECK(D->alloc(&D->bpre, (size_t)nwords));
+    if (fastpath) DCHECK(D->alloc(&D->wreg, wreg.size()));
     DCHECK(D->alloc(&D->btype, (size_t)D->nrec));
     DCHECK(D->alloc(&D->bbb, (size_t)D->nrec));
+    DCHECK(D->alloc(&D->bmiss, (size_t)D->nrec));
     DCHECK(D->alloc(&D->bprof, (size_t)D->nrec));
     uint8_t* stale_raw = nullptr;
     DCHECK(D->alloc(&stale_raw, (size_t)q * std::max<int64_t>(D->nrec, 1) * stale_sb));
@@ -434,9 +476,15 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->d_ticket, 1));
     uint8_t *g0 = nullptr, *g1 = nullptr;
     DCHECK(D->alloc(&g0, (size_t)q * D->n * sb));
-    DCHECK(D->alloc(&g1, (size_t)q * D->n * sb));
+    if (d.scheme == LBM_SCHEME_TWO_LATTICE) DCHECK(D->alloc(&g1, (size_t)q * D->n * sb));
     D->g[0] = g0;
-    D->g[1] = g1;
+    D->g[1] = d.scheme == LBM_SCHEME_TWO_LATTICE ? g1 : g0;  // AA: one lattice, updated in place
+    if (d.scheme == LBM_SCHEME_AA) {
+        // the last step of every lbm_step call writes the state snapshot here
+        DCHECK(D->alloc(&D->fpre, (size_t)q * D->n));
+        DCHECK(D->alloc(&D->rho, (size_t)D->n));
+        DCHECK(D->alloc(&D->u, (size_t)3 * D->n));
+    }
     DCHECK(D->alloc(&D->d_wall, std::max<size_t>(D->wall_nodes.size(), 1)));
     DCHECK(D->alloc(&D->d_bcnode, std::max<size_t>(D->bc_nodes.size(), 1)));
     DCHECK(D->alloc(&D->d_bcface, std::max<size_t>(D->bc_face.size(), 1)));
@@ -445,8 +493,10 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if ((rc = upload(D, D->map, map.data(), sizeof(int32_t) * map.size()))) return fail(rc);
     if ((rc = upload(D, D->bbits, bits.data(), sizeof(uint64_t) * bits.size()))) return fail(rc);
     if ((rc = upload(D, D->bpre, pre.data(), sizeof(int32_t) * pre.size()))) return fail(rc);
+    if (fastpath && (rc = upload(D, D->wreg, wreg.data(), sizeof(uint32_t) * wreg.size()))) return fail(rc);
     if ((rc = upload(D, D->btype, btype.data(), btype.size()))) return fail(rc);
     if ((rc = upload(D, D->bbb, bbb.data(), sizeof(uint32_t) * bbb.size()))) return fail(rc);
+    if ((rc = upload(D, D->bmiss, bmiss.data(), sizeof(uint32_t) * bmiss.size()))) return fail(rc);
     if ((rc = upload(D, D->bprof, bprof.data(), sizeof(double) * bprof.size()))) return fail(rc);
     if ((rc = upload(D, D->d_faces, D->faces.data(), sizeof(FaceTab) * D->faces.size()))) return fail(rc);
     if ((rc = upload(D, D->d_waves, D->waves.data(), sizeof(WaveTab) * D->waves.size()))) return fail(rc);
@@ -506,18 +556,78 @@ extern "C" int lbm_set_waveform(lbm_domain* D, int slot, int kind, double period
     }
     CHECK(hipMemcpyAsync(&D->d_waves[slot], &w, sizeof w, hipMemcpyHostToDevice, ctx->stream));
     CHECK(hipStreamSynchronize(ctx->stream));
-    D->pre_iter = -1;
+    // the two-lattice read-back recomputes the last step with the waveform
+    // table; an AA snapshot was written by that step and stays valid
+    if (D->scheme == LBM_SCHEME_TWO_LATTICE) D->pre_iter = -1;
+    return LBM_OK;
+}
+
+// Captures GRAPH_STEPS steps starting from lattice `c` (two-lattice) or from
+// an even iteration (AA) into a graph.  Kernel arguments are fixed per node
+// (the lattice parity / AA step kind alternates inside the graph); the
+// iteration counter lives in device memory, so replays need no updates.
+// Nothing captured uses a memset node.
+static int build_graph(lbm_domain* D, int c) {
+    lbm_ctx* ctx = D->ctx;
+    CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = LBM_OK;
+    for (int i = 0; i < lbm_domain::GRAPH_STEPS && rc == LBM_OK; ++i)
+        rc = launch_step(D, LBM_MODE_STEP, scheme_for(D, i), D->g[(c + i) & 1], D->g[(c + i + 1) & 1]);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    CHECK(e);
+    D->graph[c] = g;
+    CHECK(hipGraphInstantiate(&D->exec[c], g, nullptr, nullptr, 0));
+    return LBM_OK;
+}
+
+static int eager_step(lbm_domain* D, int mode) {
+    int rc = launch_step(D, mode, scheme_for(D, D->iter), D->g[D->cur], D->g[D->cur ^ 1]);
+    if (rc) return rc;
+    if (D->scheme == LBM_SCHEME_TWO_LATTICE) D->cur ^= 1;
+    D->iter++;
     return LBM_OK;
 }
 
 static int step_n(lbm_domain* D, int64_t nsteps) {
-    for (int64_t i = 0; i < nsteps; ++i) {
-        int rc = launch_step(D, LBM_MODE_STEP, D->g[D->cur], D->g[D->cur ^ 1]);
-        if (rc) return rc;
-        D->cur ^= 1;
-        D->iter++;
+    lbm_ctx* ctx = D->ctx;
+    if (nsteps <= 0) return LBM_OK;
+    const bool aa = D->scheme == LBM_SCHEME_AA;
+    // AA overwrites its only lattice, so the last step of the call also writes
+    // the pre-collision state that lbm_get_* read back.
+    int64_t body = aa ? nsteps - 1 : nsteps;
+    const int64_t G = lbm_domain::GRAPH_STEPS;
+    if (!(D->options & LBM_OPT_NO_GRAPH)) {
+        if (aa && (D->iter & 1) && body > 0) {  // AA graphs start on an even iteration
+            int rc = eager_step(D, LBM_MODE_STEP);
+            if (rc) return rc;
+            body--;
+        }
+        const int c = aa ? 0 : D->cur;
+        while (body >= G) {
+            if (!D->exec[c]) {
+                int rc = build_graph(D, c);
+                if (rc) return rc;
+            }
+            CHECK(hipGraphLaunch(D->exec[c], ctx->stream));
+            D->iter += G;  // G is even: the lattice parity is unchanged
+            body -= G;
+        }
     }
-    if (nsteps > 0) {
+    for (int64_t i = 0; i < body; ++i) {
+        int rc = eager_step(D, LBM_MODE_STEP);
+        if (rc) return rc;
+    }
+    if (aa) {
+        int rc = eager_step(D, LBM_MODE_SNAP);
+        if (rc) return rc;
+        D->state = 2;
+        D->pre_iter = D->iter;
+    } else {
         D->state = 2;
         D->pre_iter = -1;
     }
@@ -593,7 +703,8 @@ static int ensure_pre(lbm_domain* D) {
     if (rc) return rc;
     const unsigned grid = (unsigned)((D->n + 255) / 256);
     if (D->state == 2) {
-        rc = launch_step(D, LBM_MODE_PRE, D->g[D->cur ^ 1], nullptr);
+        if (D->scheme == LBM_SCHEME_AA) return lbm_ctx_fail(ctx, LBM_EIO, "AA state snapshot missing");
+        rc = launch_step(D, LBM_MODE_PRE, 0, D->g[D->cur ^ 1], nullptr);
         if (rc) return rc;
     } else {
         if (D->state == 0) hipLaunchKernelGGL(k_fill_rest, dim3(grid), dim3(256), 0, ctx->stream, D->fpre, D->n, D->q);
@@ -672,29 +783,17 @@ extern "C" int lbm_set_f(lbm_domain* D, const double* f) {
     return LBM_OK;
 }
 
+static int compute_wss(lbm_domain* D);
+
 extern "C" int lbm_get_wss(lbm_domain* D, double* wss) {
     if (!D) return LBM_EINVAL;
     lbm_ctx* ctx = D->ctx;
     if (!wss) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_get_wss: null output");
     CHECK(hipSetDevice(ctx->device));
-    int rc = ensure_pre(D);
+    int rc = compute_wss(D);
     if (rc) return rc;
-    if (!D->wss) CHECK(D->alloc(&D->wss, std::max<size_t>(D->wall_nodes.size(), 1)));
-    WssArgs a;
-    std::memset(&a, 0, sizeof a);
-    a.nwall = (int64_t)D->wall_nodes.size();
-    a.n = D->n;
-    a.wall = D->d_wall;
-    a.pnode = D->pnode;
-    a.map = D->map;
-    a.u = D->u;
-    a.wss = D->wss;
-    a.mu = 1.0 / 6.0 * (2.0 * D->tau - 1.0);
-    for (int e = 0; e < 27; ++e) a.off[e] = D->off[e];
-    CHECK(launch_wss(D->q, a, ctx->stream));
     std::vector<double> h(std::max<size_t>(D->wall_nodes.size(), 1));
-    CHECK(hipMemcpyAsync(h.data(), D->wss, sizeof(double) * D->wall_nodes.size(), hipMemcpyDeviceToHost, ctx->stream));
-    CHECK(hipStreamSynchronize(ctx->stream));
+    CHECK(hipMemcpy(h.data(), D->wss, sizeof(double) * D->wall_nodes.size(), hipMemcpyDeviceToHost));
     std::memset(wss, 0, sizeof(double) * D->nsites);
     for (size_t i = 0; i < D->wall_nodes.size(); ++i) wss[D->node_site[D->wall_nodes[i]]] = h[i];
     return LBM_OK;
@@ -716,4 +815,166 @@ extern "C" int lbm_reduce(lbm_domain* D, double* mass, double* flux) {
         for (size_t b = 0; b < D->faces.size(); ++b) flux[b] = h[1 + b];
     if (h[D->faces.size() + 1] > 0) return lbm_ctx_fail(ctx, LBM_ENAN, "non-finite density in the state");
     return LBM_OK;
+}
+
+// ------------------------------------------------------------ sub-box reads
+
+static int scatter_box(lbm_domain* D, const double* dev, int ncomp, const int32_t* o, const int32_t* bd,
+                       double* out, const std::vector<int32_t>* subset) {
+    lbm_ctx* ctx = D->ctx;
+    const int64_t nb = (int64_t)bd[0] * bd[1] * bd[2];
+    std::memset(out, 0, sizeof(double) * ncomp * nb);
+    const int nx = D->dims[0], ny = D->dims[1];
+    auto inside = [&](int64_t st, int64_t* local) {
+        const int x = (int)(st % nx), y = (int)((st / nx) % ny), z = (int)(st / ((int64_t)nx * ny));
+        const int lx = x - o[0], ly = y - o[1], lz = z - o[2];
+        if (lx < 0 || ly < 0 || lz < 0 || lx >= bd[0] || ly >= bd[1] || lz >= bd[2]) return false;
+        *local = lx + (int64_t)bd[0] * (ly + (int64_t)bd[1] * lz);
+        return true;
+    };
+    if (subset) {
+        std::vector<double> h(std::max<size_t>(subset->size(), 1));
+        if (!subset->empty())
+            CHECK(hipMemcpy(h.data(), dev, sizeof(double) * subset->size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < subset->size(); ++i) {
+            int64_t l;
+            if (inside(D->node_site[(*subset)[i]], &l)) out[l] = h[i];
+        }
+        return LBM_OK;
+    }
+    std::vector<double> h((size_t)ncomp * D->n);
+    CHECK(hipMemcpy(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int64_t m = 0; m < D->n; ++m) {
+        int64_t l;
+        if (!inside(D->node_site[m], &l)) continue;
+        for (int c = 0; c < ncomp; ++c) out[c * nb + l] = h[c * D->n + m];
+    }
+    return LBM_OK;
+}
+
+static bool box_ok(const lbm_domain* D, const int32_t* o, const int32_t* bd) {
+    if (!o || !bd) return false;
+    for (int a = 0; a < 3; ++a)
+        if (o[a] < 0 || bd[a] <= 0 || (int64_t)o[a] + bd[a] > D->dims[a]) return false;
+    return true;
+}
+
+extern "C" int lbm_get_macros_box(lbm_domain* D, const int32_t origin[3], const int32_t dims[3], double* rho,
+                                  double* u) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    if (!box_ok(D, origin, dims)) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_get_macros_box: bad box");
+    CHECK(hipSetDevice(ctx->device));
+    int rc = ensure_pre(D);
+    if (rc) return rc;
+    if (rho && (rc = scatter_box(D, D->rho, 1, origin, dims, rho, nullptr))) return rc;
+    if (u && (rc = scatter_box(D, D->u, 3, origin, dims, u, nullptr))) return rc;
+    return LBM_OK;
+}
+
+static int compute_wss(lbm_domain* D) {
+    lbm_ctx* ctx = D->ctx;
+    int rc = ensure_pre(D);
+    if (rc) return rc;
+    if (!D->wss) CHECK(D->alloc(&D->wss, std::max<size_t>(D->wall_nodes.size(), 1)));
+    WssArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.nwall = (int64_t)D->wall_nodes.size();
+    a.n = D->n;
+    a.wall = D->d_wall;
+    a.pnode = D->pnode;
+    a.map = D->map;
+    a.u = D->u;
+    a.wss = D->wss;
+    a.mu = 1.0 / 6.0 * (2.0 * D->tau - 1.0);
+    for (int e = 0; e < 27; ++e) a.off[e] = D->off[e];
+    CHECK(launch_wss(D->q, a, ctx->stream));
+    CHECK(hipStreamSynchronize(ctx->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_wss_box(lbm_domain* D, const int32_t origin[3], const int32_t dims[3], double* wss) {
+    if (!D) return LBM_EINVAL;
+    lbm_ctx* ctx = D->ctx;
+    if (!wss || !box_ok(D, origin, dims)) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_get_wss_box: bad argument");
+    CHECK(hipSetDevice(ctx->device));
+    int rc = compute_wss(D);
+    if (rc) return rc;
+    return scatter_box(D, D->wss, 1, origin, dims, wss, &D->wall_nodes);
+}
+
+// ------------------------------------------------------------------- batch
+
+extern "C" int lbm_batch_create(lbm_ctx* ctx, const lbm_domain_desc* cases, int ncases, int stack_axis,
+                                lbm_domain** out, lbm_batch_case* info) {
+    if (!ctx) return LBM_EINVAL;
+    if (!cases || ncases <= 0 || !out || stack_axis < 0 || stack_axis > 2)
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_batch_create: bad argument");
+    const lbm_domain_desc& c0 = cases[0];
+    int64_t dims[3] = {0, 0, 0};
+    int nbc = 0, nwave = 0;
+    std::vector<int> bc_base(ncases), wave_base(ncases);
+    std::vector<int64_t> base(ncases);
+    for (int c = 0; c < ncases; ++c) {
+        const lbm_domain_desc& d = cases[c];
+        if (!d.flags || d.q != c0.q || d.storage != c0.storage || d.scheme != c0.scheme ||
+            d.quirks != c0.quirks || d.tau != c0.tau || d.nbc < 0 || (d.nbc > 0 && !d.bcs))
+            return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_batch_create: case " + std::to_string(c) +
+                                                     " differs in q/storage/scheme/quirks/tau or is empty");
+        for (int a = 0; a < 3; ++a)
+            if (d.dims[a] <= 0) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_batch_create: bad dims");
+        base[c] = dims[stack_axis] + (c > 0 ? 1 : 0);
+        for (int a = 0; a < 3; ++a)
+            dims[a] = a == stack_axis ? base[c] + d.dims[a] : std::max<int64_t>(dims[a], d.dims[a]);
+        bc_base[c] = nbc;
+        nbc += d.nbc;
+        wave_base[c] = nwave;
+        int wmax = -1;
+        for (int b = 0; b < d.nbc; ++b) wmax = std::max(wmax, d.bcs[b].waveform);
+        nwave += wmax + 1;
+    }
+    if (nbc > LBM_MAXBC) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_batch_create: more than 253 boundary conditions");
+    if (nwave > LBM_MAXWAVE) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_batch_create: more than 256 waveforms");
+    const int64_t nsites = dims[0] * dims[1] * dims[2];
+    if (dims[0] > INT32_MAX / 2 || dims[1] > INT32_MAX / 2 || dims[2] > INT32_MAX / 2 || nsites > (int64_t)1 << 34)
+        return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_batch_create: packed box too large");
+    std::vector<uint8_t> flags((size_t)nsites, 0);
+    std::vector<lbm_bc_desc> bcs((size_t)std::max(nbc, 1));
+    for (int c = 0; c < ncases; ++c) {
+        const lbm_domain_desc& d = cases[c];
+        int64_t o[3] = {0, 0, 0};
+        o[stack_axis] = base[c];
+        for (int k = 0; k < d.dims[2]; ++k)
+            for (int j = 0; j < d.dims[1]; ++j)
+                for (int i = 0; i < d.dims[0]; ++i) {
+                    const uint8_t f = d.flags[i + (int64_t)d.dims[0] * (j + (int64_t)d.dims[1] * k)];
+                    if (f >= 3 && f - 3 >= d.nbc)
+                        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_batch_create: flag refers to a missing BC");
+                    const int64_t st = (i + o[0]) + dims[0] * ((j + o[1]) + dims[1] * (k + o[2]));
+                    flags[st] = f >= 3 ? (uint8_t)(f + bc_base[c]) : f;
+                }
+        for (int b = 0; b < d.nbc; ++b) {
+            lbm_bc_desc bc = d.bcs[b];
+            if (bc.waveform >= 0) bc.waveform += wave_base[c];
+            // profile centres are in case coordinates: shift them
+            int t = 0;
+            for (int a = 0; a < 3; ++a)
+                if (a != bc.axis) bc.center[t++] += (double)o[a];
+            bcs[bc_base[c] + b] = bc;
+        }
+        if (info) {
+            for (int a = 0; a < 3; ++a) {
+                info[c].origin[a] = (int32_t)o[a];
+                info[c].dims[a] = d.dims[a];
+            }
+            info[c].bc_base = bc_base[c];
+            info[c].wave_base = wave_base[c];
+        }
+    }
+    lbm_domain_desc all = c0;
+    for (int a = 0; a < 3; ++a) all.dims[a] = (int32_t)dims[a];
+    all.flags = flags.data();
+    all.bcs = bcs.data();
+    all.nbc = nbc;
+    return lbm_domain_create(ctx, &all, out);
 }

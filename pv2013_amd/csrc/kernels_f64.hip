@@ -6,15 +6,14 @@
 
 using namespace lbm;
 
-hipError_t launch_step_f64(int mode, const StepArgs& a, int block, hipStream_t s) {
-    if (a.q == 19) return launch_step_t<19, PrecF64>(mode, a, block, s);
-    return launch_step_t<27, PrecF64>(mode, a, block, s);
+hipError_t launch_step_f64(int mode, int sch, const StepArgs& a, hipStream_t s) {
+    if (a.q == 19) return launch_step_t<19, PrecF64>(mode, sch, a, s);
+    return launch_step_t<27, PrecF64>(mode, sch, a, s);
 }
 
-hipError_t launch_init_f64(int q, const InitArgs& a, int block, hipStream_t s) {
-    (void)block;
-    if (q == 19) return launch_init_t<19, PrecF64>(a, s);
-    return launch_init_t<27, PrecF64>(a, s);
+hipError_t launch_init_f64(int q, bool swap, const InitArgs& a, hipStream_t s) {
+    if (q == 19) return launch_init_t<19, PrecF64>(a, swap, s);
+    return launch_init_t<27, PrecF64>(a, swap, s);
 }
 
 // Simulation.cpp:1374-1421: mean over present neighbours x + c_d (the rest

@@ -4,21 +4,20 @@
 
 using namespace lbm;
 
-hipError_t launch_step_lp(int storage, int mode, const StepArgs& a, int block, hipStream_t s) {
+hipError_t launch_step_lp(int storage, int mode, int sch, const StepArgs& a, hipStream_t s) {
     if (storage == LBM_STORE_F32) {
-        if (a.q == 19) return launch_step_t<19, PrecF32S>(mode, a, block, s);
-        return launch_step_t<27, PrecF32S>(mode, a, block, s);
+        if (a.q == 19) return launch_step_t<19, PrecF32S>(mode, sch, a, s);
+        return launch_step_t<27, PrecF32S>(mode, sch, a, s);
     }
-    if (a.q == 19) return launch_step_t<19, PrecF16S>(mode, a, block, s);
-    return launch_step_t<27, PrecF16S>(mode, a, block, s);
+    if (a.q == 19) return launch_step_t<19, PrecF16S>(mode, sch, a, s);
+    return launch_step_t<27, PrecF16S>(mode, sch, a, s);
 }
 
-hipError_t launch_init_lp(int storage, int q, const InitArgs& a, int block, hipStream_t s) {
-    (void)block;
+hipError_t launch_init_lp(int storage, int q, bool swap, const InitArgs& a, hipStream_t s) {
     if (storage == LBM_STORE_F32) {
-        if (q == 19) return launch_init_t<19, PrecF32S>(a, s);
-        return launch_init_t<27, PrecF32S>(a, s);
+        if (q == 19) return launch_init_t<19, PrecF32S>(a, swap, s);
+        return launch_init_t<27, PrecF32S>(a, swap, s);
     }
-    if (q == 19) return launch_init_t<19, PrecF16S>(a, s);
-    return launch_init_t<27, PrecF16S>(a, s);
+    if (q == 19) return launch_init_t<19, PrecF16S>(a, swap, s);
+    return launch_init_t<27, PrecF16S>(a, swap, s);
 }

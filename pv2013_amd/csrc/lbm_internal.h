@@ -9,8 +9,8 @@
 
 #include "lbm_capi.h"
 
-constexpr int LBM_MAXBC = 32;
-constexpr int LBM_MAXWAVE = 16;
+constexpr int LBM_MAXBC = 253;   // flag values 3 .. 255
+constexpr int LBM_MAXWAVE = 256;
 constexpr int LBM_MAXK = 16;
 
 // One Zou-He boundary condition as the kernels see it.  The population index
@@ -39,9 +39,12 @@ struct StepArgs {
     const int32_t* pnode;   // padded map index of node n
     const int32_t* map;     // padded site -> node id or -1
     const uint64_t* bbits;  // bit n: node n has a boundary record
+    const uint32_t* wreg;   // bit w: the 64 nodes of wavefront w are interior nodes of one
+                            // contiguous row (null: no fast path)
     const int32_t* bpre;    // records before word n/64
     const uint8_t* btype;   // per record: flag value (1 fluid, 2 wall, 3+b)
     const uint32_t* bbb;    // per record: populations taken by bounce-back
+    const uint32_t* bmiss;  // per record: populations whose source node is missing
     const double* bprof;    // per record: velocity profile factor
     const void* bstale;     // [e][r] stale populations (double, or float deviations)
     const FaceTab* faces;
@@ -78,18 +81,19 @@ struct WssArgs {
     double* wss;           // per wall node
 };
 
-enum { LBM_MODE_STEP = 0, LBM_MODE_PRE = 1 };
+enum { LBM_MODE_STEP = 0, LBM_MODE_PRE = 1, LBM_MODE_SNAP = 2 };
 
 // kernels_f64.hip
-hipError_t launch_step_f64(int mode, const StepArgs& a, int block, hipStream_t s);
-hipError_t launch_init_f64(int q, const InitArgs& a, int block, hipStream_t s);
+// sch: 0 two-lattice pull, 1 AA gather step, 2 AA local step
+hipError_t launch_step_f64(int mode, int sch, const StepArgs& a, hipStream_t s);
+hipError_t launch_init_f64(int q, bool swap, const InitArgs& a, hipStream_t s);
 hipError_t launch_wss(int q, const WssArgs& a, hipStream_t s);
 hipError_t launch_reduce(const double* rho, const double* u, int64_t n, const int32_t* bnode,
                          const int32_t* bface, int64_t nbnd, const FaceTab* faces, int nbc,
                          double* out /* 1 + nbc + 1 */, hipStream_t s);
 // kernels_lp.hip (storage LBM_STORE_F32 or LBM_STORE_F16S)
-hipError_t launch_step_lp(int storage, int mode, const StepArgs& a, int block, hipStream_t s);
-hipError_t launch_init_lp(int storage, int q, const InitArgs& a, int block, hipStream_t s);
+hipError_t launch_step_lp(int storage, int mode, int sch, const StepArgs& a, hipStream_t s);
+hipError_t launch_init_lp(int storage, int q, bool swap, const InitArgs& a, hipStream_t s);
 
 // Grid rounded up to whole groups of 8 blocks so the XCD-aware block remap is a
 // bijection (blocks are dispatched round-robin over the 8 XCDs).

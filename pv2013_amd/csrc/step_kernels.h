@@ -355,9 +355,48 @@ __device__ __forceinline__ typename P::calc_t load_pop(const typename P::store_t
     return (typename P::calc_t)g[i];
 }
 
+// Streaming schemes.
+//   SCH_TWO:       pull from g_in, write g_out (two lattices).
+//   SCH_AA_GATHER: one lattice A; read A[opp d][x - c_d], write A[d][x + c_d]
+//                  (a missing destination keeps the value at A[opp d][x], where
+//                  the next local step reads it as the bounced-back population).
+//   SCH_AA_LOCAL:  read A[d][x], write A[opp d][x].
+// Every slot of A is read and written by one node only, so both AA steps run
+// in place without races.
+enum { SCH_TWO = 0, SCH_AA_GATHER = 1, SCH_AA_LOCAL = 2 };
+
+template <class P, int Q>
+__device__ __forceinline__ void write_pre(const StepArgs& a, int64_t n, const typename P::calc_t* f) {
+    using L = Lat<Q>;
+    using T = typename P::calc_t;
+    constexpr bool F64 = sizeof(T) == 8;
+    const int64_t N = a.n;
+    double fd[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) fd[d] = F64 ? (double)f[d] : L::w(d) + (double)f[d];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) a.fpre[(int64_t)d * N + n] = fd[d];
+    double rho, u[3];
+    if constexpr (F64) {
+        moments_f64<Q>(fd, rho, u);
+    } else {
+        double dr = 0.0, j[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            const double v = (double)f[d];
+            dr += v;
+            j[0] += L::c(d, 0) * v; j[1] += L::c(d, 1) * v; j[2] += L::c(d, 2) * v;
+        }
+        rho = 1.0 + dr;
+        u[0] = j[0] / rho; u[1] = j[1] / rho; u[2] = j[2] / rho;
+    }
+    a.rho[n] = rho;
+    a.u[n] = u[0]; a.u[N + n] = u[1]; a.u[2 * N + n] = u[2];
+}
+
 // ------------------------------------------------------------------- kernel
 
-template <int Q, class P, int MODE>
+template <int Q, class P, int MODE, int SCH>
 __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     using L = Lat<Q>;
     using S = typename P::store_t;
@@ -368,89 +407,115 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     if (lb < a.nblocks && n < a.n) {
         const S* __restrict__ gin = (const S*)a.gin;
         const int64_t N = a.n;
-        const int32_t p = a.pnode[n];
         T f[Q];
+        int32_t dst[Q];  // AA gather: destination node of each population (-1: none)
+        // Wavefront index, made wave-uniform so the loads below that depend
+        // only on it are scalar loads.
+        const int wv = __builtin_amdgcn_readfirstlane((int)(n >> 6));
+        const bool reg = a.wreg && ((a.wreg[wv >> 5] >> (wv & 31)) & 1u);
+        int32_t p = 0;
         uint32_t miss = 0;
+        if (reg) {
+            // All 64 lanes are interior nodes of one contiguous row, so for
+            // every direction their sources (and destinations) are one
+            // contiguous run of node ids.
+            const int32_t p0 = a.pnode[(int64_t)wv << 6];
+            const int lane = (int)(n & 63);
 #pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            if (d == L::rest) {
-                f[d] = load_pop<P>(gin, (int64_t)d * N + n);
-            } else {
-                const int32_t src = a.map[p - a.off[d]];
-                if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)d * N + src);
-                else { f[d] = T(0); miss |= 1u << d; }
-            }
-        }
-        const uint64_t word = a.bbits[n >> 6];
-        const int bit = (int)(n & 63);
-        if ((word >> bit) & 1ull) {
-            const int64_t r = a.bpre[n >> 6] + __popcll(word & ((1ull << bit) - 1ull));
-            const uint32_t bb = a.bbb[r];
-            if (miss) {
-                const T* stale = (const T*)a.bstale;
-#pragma unroll
-                for (int d = 0; d < Q; ++d) {
-                    if (!((miss >> d) & 1u)) continue;
-                    if ((bb >> d) & 1u) f[d] = load_pop<P>(gin, (int64_t)L::opp(d) * N + n);
-                    else f[d] = stale[(int64_t)d * a.nrec + r];
+            for (int d = 0; d < Q; ++d) {
+                if (SCH == SCH_AA_LOCAL || d == L::rest) {
+                    f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_LOCAL ? d : d) * N + n);
+                    dst[d] = (int32_t)n;
+                } else {
+                    const int64_t src = (int64_t)a.map[p0 - a.off[d]] + lane;
+                    f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
+                    if (SCH == SCH_AA_GATHER) dst[d] = a.map[p0 + a.off[d]] + lane;
                 }
             }
-            const int type = a.btype[r];
-            if (type >= 3) {
-                const FaceTab& F = a.faces[type - 3];
-                double uv[3] = {0.0, 0.0, 0.0};
-                if (F.kind == LBM_BC_VELOCITY) {
-                    double w = 1.0;
-                    if (F.waveform >= 0) {
-                        const int64_t it = *a.iter - (MODE == LBM_MODE_PRE ? 1 : 0);
-                        w = wave_value(a.waves[F.waveform], it);
-                    }
-                    const double fac = a.bprof[r] * w;
-                    uv[0] = F.u[0] * fac; uv[1] = F.u[1] * fac; uv[2] = F.u[2] * fac;
-                }
-                ZouHe<Q, T>::apply(F, uv, f);
-            }
-        }
-        if (MODE == LBM_MODE_STEP) {
-            S* __restrict__ gout = (S*)a.gout;
-            if constexpr (F64) {
-                double g[Q];
-                collide_f64<Q>((const double*)f, a.omega, g);
-#pragma unroll
-                for (int d = 0; d < Q; ++d) gout[(int64_t)d * N + n] = (S)g[d];
-            } else {
-                collide_dev<Q>((float*)f, (float)a.omega);
-#pragma unroll
-                for (int d = 0; d < Q; ++d) gout[(int64_t)d * N + n] = (S)f[d];
-            }
+            if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
         } else {
-            double fd[Q];
+            p = a.pnode[n];
 #pragma unroll
-            for (int d = 0; d < Q; ++d) fd[d] = F64 ? (double)f[d] : L::w(d) + (double)f[d];
-#pragma unroll
-            for (int d = 0; d < Q; ++d) a.fpre[(int64_t)d * N + n] = fd[d];
-            double rho, u[3];
-            if constexpr (F64) {
-                moments_f64<Q>(fd, rho, u);
-            } else {
-                double dr = 0.0, j[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-                for (int d = 0; d < Q; ++d) {
-                    const double v = (double)f[d];
-                    dr += v;
-                    j[0] += L::c(d, 0) * v; j[1] += L::c(d, 1) * v; j[2] += L::c(d, 2) * v;
+            for (int d = 0; d < Q; ++d) {
+                if (SCH == SCH_AA_LOCAL || d == L::rest) {
+                    f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+                } else {
+                    const int32_t src = a.map[p - a.off[d]];
+                    if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
+                    else { f[d] = T(0); miss |= 1u << d; }
+                    if (SCH == SCH_AA_GATHER) dst[d] = a.map[p + a.off[d]];
                 }
-                rho = 1.0 + dr;
-                u[0] = j[0] / rho; u[1] = j[1] / rho; u[2] = j[2] / rho;
             }
-            a.rho[n] = rho;
-            a.u[n] = u[0]; a.u[N + n] = u[1]; a.u[2 * N + n] = u[2];
+            if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
+            const uint64_t word = a.bbits[n >> 6];
+            const int bit = (int)(n & 63);
+            if ((word >> bit) & 1ull) {
+                const int64_t r = a.bpre[n >> 6] + __popcll(word & ((1ull << bit) - 1ull));
+                const uint32_t bb = a.bbb[r];
+                if (SCH == SCH_AA_LOCAL) miss = a.bmiss[r];
+                if (miss) {
+                    const T* stale = (const T*)a.bstale;
+#pragma unroll
+                    for (int d = 0; d < Q; ++d) {
+                        if (!((miss >> d) & 1u)) continue;
+                        if ((bb >> d) & 1u) {
+                            // bounce-back: this node's own post-collision g[opp d]
+                            if (SCH == SCH_TWO) f[d] = load_pop<P>(gin, (int64_t)L::opp(d) * N + n);
+                            if (SCH == SCH_AA_GATHER) f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+                            // SCH_AA_LOCAL: A[d][x] already holds it (kept by the gather step)
+                        } else {
+                            f[d] = stale[(int64_t)d * a.nrec + r];
+                        }
+                    }
+                }
+                const int type = a.btype[r];
+                if (type >= 3) {
+                    const FaceTab& F = a.faces[type - 3];
+                    double uv[3] = {0.0, 0.0, 0.0};
+                    if (F.kind == LBM_BC_VELOCITY) {
+                        double w = 1.0;
+                        if (F.waveform >= 0) {
+                            const int64_t it = *a.iter - (MODE == LBM_MODE_PRE ? 1 : 0);
+                            w = wave_value(a.waves[F.waveform], it);
+                        }
+                        const double fac = a.bprof[r] * w;
+                        uv[0] = F.u[0] * fac; uv[1] = F.u[1] * fac; uv[2] = F.u[2] * fac;
+                    }
+                    ZouHe<Q, T>::apply(F, uv, f);
+                }
+            }
+        }
+        if (MODE == LBM_MODE_PRE || MODE == LBM_MODE_SNAP) write_pre<P, Q>(a, n, f);
+        if (MODE != LBM_MODE_PRE) {
+            S* __restrict__ gout = (S*)a.gout;
+            T g[Q];
+            if constexpr (F64) {
+                collide_f64<Q>((const double*)f, a.omega, (double*)g);
+            } else {
+#pragma unroll
+                for (int d = 0; d < Q; ++d) g[d] = f[d];
+                collide_dev<Q>((float*)g, (float)a.omega);
+            }
+#pragma unroll
+            for (int d = 0; d < Q; ++d) {
+                if (SCH == SCH_TWO) {
+                    gout[(int64_t)d * N + n] = (S)g[d];
+                } else if (SCH == SCH_AA_LOCAL) {
+                    gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
+                } else {
+                    if (dst[d] >= 0) gout[(int64_t)d * N + dst[d]] = (S)g[d];
+                    else gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
+                }
+            }
         }
     }
-    if (MODE == LBM_MODE_STEP) finish_launch(a.iter, a.ticket);
+    if (MODE != LBM_MODE_PRE) finish_launch(a.iter, a.ticket);
 }
 
-template <int Q, class P>
+// Initial lattice from a given pre-collision state (or f = w when fpre is
+// null): collide once and store in the layout the first step reads (swapped
+// for AA, whose first step is a gather).
+template <int Q, class P, bool SWAP>
 __global__ __launch_bounds__(256) void k_init(InitArgs a) {
     using L = Lat<Q>;
     using S = typename P::store_t;
@@ -459,53 +524,59 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a) {
     const int64_t n = lb * blockDim.x + threadIdx.x;
     if (lb >= a.nblocks || n >= a.n) return;
     S* gout = (S*)a.gout;
+    T g[Q];
     if constexpr (sizeof(T) == 8) {
-        double f[Q], g[Q];
+        double f[Q];
 #pragma unroll
         for (int d = 0; d < Q; ++d) f[d] = a.fpre ? a.fpre[(int64_t)d * a.n + n] : L::w(d);
-        collide_f64<Q>(f, a.omega, g);
-#pragma unroll
-        for (int d = 0; d < Q; ++d) gout[(int64_t)d * a.n + n] = (S)g[d];
+        collide_f64<Q>(f, a.omega, (double*)g);
     } else {
-        float f[Q];
 #pragma unroll
-        for (int d = 0; d < Q; ++d) f[d] = a.fpre ? (float)(a.fpre[(int64_t)d * a.n + n] - L::w(d)) : 0.f;
-        collide_dev<Q>(f, (float)a.omega);
-#pragma unroll
-        for (int d = 0; d < Q; ++d) gout[(int64_t)d * a.n + n] = (S)f[d];
+        for (int d = 0; d < Q; ++d) g[d] = a.fpre ? (float)(a.fpre[(int64_t)d * a.n + n] - L::w(d)) : 0.f;
+        collide_dev<Q>((float*)g, (float)a.omega);
     }
+#pragma unroll
+    for (int d = 0; d < Q; ++d) gout[(int64_t)(SWAP ? L::opp(d) : d) * a.n + n] = (S)g[d];
 }
 
 // Every pointer a launch dereferences must be set; a missing one is an error
 // returned to the caller, never a launch.
-inline bool step_args_ok(int mode, const StepArgs& a) {
+inline bool step_args_ok(int mode, int sch, const StepArgs& a) {
     if (!a.pnode || !a.map || !a.bbits || !a.bpre || !a.gin || !a.faces || !a.waves || !a.iter) return false;
-    if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale)) return false;
-    if (mode == LBM_MODE_STEP) return a.gout && a.ticket;
-    return a.fpre && a.rho && a.u;
+    if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale || !a.bmiss)) return false;
+    if (mode != LBM_MODE_PRE && (!a.gout || !a.ticket)) return false;
+    if (mode != LBM_MODE_STEP && (!a.fpre || !a.rho || !a.u)) return false;
+    if (mode == LBM_MODE_PRE && sch != SCH_TWO) return false;
+    return sch >= SCH_TWO && sch <= SCH_AA_LOCAL;
+}
+
+template <int Q, class P, int MODE>
+void launch_mode(int sch, int grid, const StepArgs& b, hipStream_t s) {
+    if (sch == SCH_TWO) hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_TWO>), dim3(grid), dim3(256), 0, s, b);
+    else if (sch == SCH_AA_GATHER) hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_AA_GATHER>), dim3(grid), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_AA_LOCAL>), dim3(grid), dim3(256), 0, s, b);
 }
 
 template <int Q, class P>
-hipError_t launch_step_t(int mode, const StepArgs& a, int block, hipStream_t s) {
-    if (!step_args_ok(mode, a)) return hipErrorInvalidValue;
+hipError_t launch_step_t(int mode, int sch, const StepArgs& a, hipStream_t s) {
+    if (!step_args_ok(mode, sch, a)) return hipErrorInvalidValue;
     const int grid = lbm_grid_blocks(a.n, 256);
     StepArgs b = a;
     b.nblocks = (int)((a.n + 255) / 256);
-    (void)block;
-    if (mode == LBM_MODE_STEP)
-        hipLaunchKernelGGL((k_step<Q, P, LBM_MODE_STEP>), dim3(grid), dim3(256), 0, s, b);
-    else
-        hipLaunchKernelGGL((k_step<Q, P, LBM_MODE_PRE>), dim3(grid), dim3(256), 0, s, b);
+    if (mode == LBM_MODE_STEP) launch_mode<Q, P, LBM_MODE_STEP>(sch, grid, b, s);
+    else if (mode == LBM_MODE_SNAP) launch_mode<Q, P, LBM_MODE_SNAP>(sch, grid, b, s);
+    else hipLaunchKernelGGL((k_step<Q, P, LBM_MODE_PRE, SCH_TWO>), dim3(grid), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
 template <int Q, class P>
-hipError_t launch_init_t(const InitArgs& a, hipStream_t s) {
+hipError_t launch_init_t(const InitArgs& a, bool swap, hipStream_t s) {
     if (!a.gout) return hipErrorInvalidValue;
     const int grid = lbm_grid_blocks(a.n, 256);
     InitArgs b = a;
     b.nblocks = (int)((a.n + 255) / 256);
-    hipLaunchKernelGGL((k_init<Q, P>), dim3(grid), dim3(256), 0, s, b);
+    if (swap) hipLaunchKernelGGL((k_init<Q, P, true>), dim3(grid), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL((k_init<Q, P, false>), dim3(grid), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

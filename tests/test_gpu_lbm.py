@@ -29,9 +29,9 @@ def bits_equal(a, b):
     return np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64))
 
 
-def both(P, oracle, ctx, q, dims, flags, tau, bcs, quirks=1, storage=0, fast_axis=-1):
+def both(P, oracle, ctx, q, dims, flags, tau, bcs, quirks=1, storage=0, fast_axis=-1, options=0):
     dom = P.Domain(ctx, dims, flags, tau, [P.make_bc(**b) for b in bcs], q=q, storage=storage,
-                   quirks=quirks, fast_axis=fast_axis)
+                   quirks=quirks, fast_axis=fast_axis, options=options)
     ref = oracle.Sim(q, dims, flags, tau, [oracle.make_bc(**b) for b in bcs], quirks=quirks)
     return dom, ref
 
@@ -89,15 +89,16 @@ def test_fp64_bitwise_all_faces(P, oracle, ctx, q, quirks):
     assert bits_equal(dom.wss(), ref.wss())
 
 
-@pytest.mark.parametrize("fast_axis", [-1, 0, 1])
-def test_fp64_bitwise_pipe_config1(P, oracle, ctx, fast_axis):
+@pytest.mark.parametrize("fast_axis,options", [(-1, 0), (0, 0), (1, 0), (-1, 1), (-1, 2)])
+def test_fp64_bitwise_pipe_config1(P, oracle, ctx, fast_axis, options):
     """D3Q19 FP64, 32x32x128 z-pipe of radius 12, tau 0.8, parabolic inlet
-    u_max 0.05, outlet rho 1.0, 2000 steps from rest (the small plain case)."""
+    u_max 0.05, outlet rho 1.0, 2000 steps from rest (the small plain case);
+    shorter runs for the other node orders and with the row fast path off."""
     dims = (32, 32, 128)
     cl, bcs = pipe_case(oracle, dims, 12.0, 0.05)
-    steps = 2000 if fast_axis == -1 else 200
-    print("pipe 32x32x128 fast_axis=%d %d steps" % (fast_axis, steps), flush=True)
-    dom, ref = both(P, oracle, ctx, 19, dims, cl, 0.8, bcs, fast_axis=fast_axis)
+    steps = 2000 if (fast_axis, options) == (-1, 0) else 200
+    print("pipe 32x32x128 fast_axis=%d options=%d %d steps" % (fast_axis, options, steps), flush=True)
+    dom, ref = both(P, oracle, ctx, 19, dims, cl, 0.8, bcs, fast_axis=fast_axis, options=options)
     dom.step(steps)
     ref.step(steps)
     assert bits_equal(dom.f(), ref.f())
@@ -151,3 +152,68 @@ def test_mass_conserved_closed_box_clean(P, oracle, ctx):
         dom.step(200)
         m1, _ = dom.reduce()
         assert abs(m1 - m0) < tol * m0
+
+
+def test_batch_matches_individual_runs(P, oracle, ctx):
+    """Three pipes of different sizes, one with a Fourier inlet, packed into one
+    domain: every case is bit-identical to the same case run alone."""
+    specs = [((14, 14, 40), 5.5, 0.03, -1), ((20, 18, 33), 7.5, 0.02, 0), ((12, 12, 57), 4.5, 0.04, -1)]
+    cases, singles = [], []
+    rng = np.random.default_rng(5)
+    a, phi = 0.3 * rng.random(4), 2 * np.pi * rng.random(4)
+    for dims, R, um, wf in specs:
+        cl, bcs = pipe_case(oracle, dims, R, um, waveform=wf)
+        pb = [P.make_bc(**b) for b in bcs]
+        cases.append(dict(dims=dims, flags=cl, tau=0.7, bcs=pb, q=19, storage=P.STORE_F64))
+        d = P.Domain(ctx, dims, cl, 0.7, pb)
+        if wf >= 0:
+            d.set_waveform(wf, P.WAVE_FOURIER, 300.0, a, phi)
+        singles.append(d)
+    dom, info = P.batch(ctx, cases)
+    dom.set_waveform(info[1]["wave_base"] + 0, P.WAVE_FOURIER, 300.0, a, phi)
+    dom.step(150)
+    for d, inf in zip(singles, info):
+        d.step(150)
+        r1, u1 = d.macros()
+        r2, u2 = dom.macros_box(inf["origin"], inf["dims"])
+        assert bits_equal(r1, r2) and bits_equal(u1, u2)
+        assert bits_equal(d.wss(), dom.wss_box(inf["origin"], inf["dims"]))
+
+
+@pytest.mark.parametrize("storage", [0, 1, 2])
+def test_aa_equals_two_lattice(P, oracle, ctx, storage):
+    """The AA scheme (one lattice, alternating gather / local steps) computes
+    exactly what the two-lattice pull computes, in every precision; with the
+    graph path, the eager path, odd and even step counts."""
+    dims = (16, 16, 40)
+    cl, bcs = pipe_case(oracle, dims, 6.0, 0.03, waveform=0)
+    bc = [P.make_bc(**b) for b in bcs]
+    a, phi = np.array([0.2, 0.1]), np.array([0.3, 1.1])
+    two = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage)
+    aa = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA)
+    aa_eager = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA, options=P.capi.OPT_NO_GRAPH)
+    for d in (two, aa, aa_eager):
+        d.set_waveform(0, P.WAVE_FOURIER, 200.0, a, phi)
+    for n in (1, 70, 3, 130):
+        for d in (two, aa, aa_eager):
+            d.step(n)
+        f2 = two.f()
+        assert bits_equal(aa.f(), f2) and bits_equal(aa_eager.f(), f2), "after %d" % two.iteration
+        r2, u2 = two.macros()
+        r1, u1 = aa.macros()
+        assert bits_equal(r1, r2) and bits_equal(u1, u2)
+
+
+def test_aa_fp64_matches_oracle_six_faces(P, oracle, ctx):
+    dims = (10, 11, 12)
+    cl, bcs = six_face_box(oracle, dims)
+    dom = P.Domain(ctx, dims, cl, 0.7, [P.make_bc(**b) for b in bcs], q=27, scheme=P.SCHEME_AA)
+    ref = oracle.Sim(27, dims, cl, 0.7, [oracle.make_bc(**b) for b in bcs], quirks=1)
+    f0 = near_eq_state(oracle, 27, cl, np.random.default_rng(11))
+    dom.set_f(f0)
+    ref.set_f(f0)
+    for n in (1, 2, 17):
+        dom.step(n)
+        ref.step(n)
+        assert bits_equal(dom.f(), ref.f())
+    assert bits_equal(dom.wss(), ref.wss())

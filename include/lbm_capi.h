@@ -180,6 +180,25 @@ int lbm_get_wss(lbm_domain* dom, double* wss);
  * its face (flux has nbc entries, may be NULL).  LBM_ENAN on a non-finite state. */
 int lbm_reduce(lbm_domain* dom, double* mass, double* flux);
 
+/* ---------------------------------------------------------------- autotune */
+
+typedef struct {
+    int32_t scheme;     /* best LBM_SCHEME_* */
+    int32_t fast_axis;  /* best node-order axis */
+    int32_t options;    /* best LBM_OPT_* bits */
+    int32_t ntried;     /* candidates timed (0 when read from the cache) */
+    double mlups;       /* fluid-node MLUPS of the best candidate */
+    uint64_t key;       /* case signature used in the cache */
+} lbm_tune_result;
+
+/* Times each candidate layout of `desc` (streaming scheme two-lattice / AA,
+ * node-order fast axis x / y / z, row fast path on / off) for `steps` steps
+ * after a warm-up and returns the fastest.  When cache_path is not NULL the
+ * result is looked up there first (key: lattice, storage, dims, flag hash) and
+ * appended after tuning.  No reference counterpart. */
+int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps, const char* cache_path,
+                 lbm_tune_result* out);
+
 /* ------------------------------------------------------------------- batch */
 
 typedef struct {

@@ -6,12 +6,12 @@ geometry generators.
 """
 from .capi import (BC_PRESSURE, BC_VELOCITY, PROFILE_PARABOLIC, PROFILE_UNIFORM, SCHEME_AA,
                    SCHEME_TWO_LATTICE, STORE_F16S, STORE_F32, STORE_F64, WAVE_FOURIER, WAVE_NONE,
-                   WAVE_PV2013, Context, Domain, LbmError, batch, grid_from_bbox, lib, make_bc,
+                   WAVE_PV2013, Context, Domain, LbmError, autotune, batch, grid_from_bbox, lib, make_bc,
                    make_grid, make_plane, read_stl)
 
 __all__ = [
     "BC_PRESSURE", "BC_VELOCITY", "PROFILE_PARABOLIC", "PROFILE_UNIFORM", "SCHEME_AA",
     "SCHEME_TWO_LATTICE", "STORE_F16S", "STORE_F32", "STORE_F64", "WAVE_FOURIER", "WAVE_NONE",
-    "WAVE_PV2013", "Context", "Domain", "LbmError", "batch", "grid_from_bbox", "lib", "make_bc", "make_grid",
+    "WAVE_PV2013", "Context", "Domain", "LbmError", "autotune", "batch", "grid_from_bbox", "lib", "make_bc", "make_grid",
     "make_plane", "read_stl",
 ]

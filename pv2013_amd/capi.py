@@ -27,7 +27,7 @@ EXPORTS = (
     "lbm_read_stl_binary", "lbm_grid_from_bbox", "lbm_voxelize", "lbm_classify", "lbm_domain_create",
     "lbm_domain_destroy", "lbm_domain_info", "lbm_set_waveform", "lbm_step", "lbm_step_timed",
     "lbm_sync", "lbm_iteration", "lbm_get_f", "lbm_set_f", "lbm_get_macros", "lbm_get_wss",
-    "lbm_reduce", "lbm_get_macros_box", "lbm_get_wss_box", "lbm_batch_create",
+    "lbm_reduce", "lbm_get_macros_box", "lbm_get_wss_box", "lbm_batch_create", "lbm_autotune",
 )
 
 
@@ -59,6 +59,11 @@ class DomainDesc(C.Structure):
                 ("scheme", C.c_int32), ("quirks", C.c_int32), ("block", C.c_int32),
                 ("tau", C.c_double), ("flags", C.c_void_p), ("bcs", C.c_void_p),
                 ("nbc", C.c_int32), ("fast_axis", C.c_int32), ("options", C.c_int32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("scheme", C.c_int32), ("fast_axis", C.c_int32), ("options", C.c_int32),
+                ("ntried", C.c_int32), ("mlups", C.c_double), ("key", C.c_uint64)]
 
 
 class BatchCase(C.Structure):
@@ -107,6 +112,7 @@ def lib():
         L.lbm_get_macros_box.argtypes = [vp, vp, vp, vp, vp]
         L.lbm_get_wss_box.argtypes = [vp, vp, vp, vp]
         L.lbm_batch_create.argtypes = [vp, vp, i32, i32, P(vp), vp]
+        L.lbm_autotune.argtypes = [vp, P(DomainDesc), i32, C.c_char_p, P(TuneResult)]
         _lib = L
     return _lib
 
@@ -367,3 +373,14 @@ def batch(ctx, cases, stack_axis=1):
     dom = Domain(ctx, dims, None, cases[0]["tau"], [None] * nbc, q=cases[0].get("q", 19),
                  storage=cases[0].get("storage", STORE_F64), _handle=h)
     return dom, out
+
+
+def autotune(ctx, steps=200, cache_path=None, **domain_kw):
+    """Fastest layout for a case (lbm_autotune).  Returns a dict with scheme,
+    fast_axis, options, mlups, ntried and the cache key."""
+    d, keep = domain_desc(**domain_kw)
+    r = TuneResult()
+    ctx.check(lib().lbm_autotune(ctx.h, C.byref(d), steps,
+                                 cache_path.encode() if cache_path else None, C.byref(r)), "lbm_autotune")
+    return {"scheme": r.scheme, "fast_axis": r.fast_axis, "options": r.options, "mlups": r.mlups,
+            "ntried": r.ntried, "key": "%016x" % r.key}

@@ -978,3 +978,84 @@ extern "C" int lbm_batch_create(lbm_ctx* ctx, const lbm_domain_desc* cases, int 
     all.nbc = nbc;
     return lbm_domain_create(ctx, &all, out);
 }
+
+// --------------------------------------------------------------- autotuner
+
+static uint64_t case_key(const lbm_domain_desc* d) {
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over the case
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < n; ++i) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    };
+    mix(d->dims, sizeof d->dims);
+    mix(&d->q, sizeof d->q);
+    mix(&d->storage, sizeof d->storage);
+    mix(&d->quirks, sizeof d->quirks);
+    mix(d->flags, (size_t)d->dims[0] * d->dims[1] * d->dims[2]);
+    return h;
+}
+
+extern "C" int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps, const char* cache_path,
+                            lbm_tune_result* out) {
+    if (!ctx) return LBM_EINVAL;
+    if (!desc || !out || !desc->flags || steps <= 0) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_autotune: bad argument");
+    std::memset(out, 0, sizeof *out);
+    const uint64_t key = case_key(desc);
+    out->key = key;
+    if (cache_path) {
+        if (FILE* fp = std::fopen(cache_path, "r")) {
+            unsigned long long k;
+            int sch, ax, opt;
+            double ml;
+            while (std::fscanf(fp, "%llx %d %d %d %lf", &k, &sch, &ax, &opt, &ml) == 5) {
+                if (k == key) {
+                    out->scheme = sch;
+                    out->fast_axis = ax;
+                    out->options = opt;
+                    out->mlups = ml;
+                    std::fclose(fp);
+                    return LBM_OK;
+                }
+            }
+            std::fclose(fp);
+        }
+    }
+    double best = -1.0;
+    for (int sch = 0; sch < 2; ++sch)
+        for (int ax = 0; ax < 3; ++ax)
+            for (int fp = 0; fp < 2; ++fp) {
+                lbm_domain_desc d = *desc;
+                d.scheme = sch;
+                d.fast_axis = ax;
+                d.options = (desc->options & ~LBM_OPT_NO_ROW_FASTPATH) | (fp ? LBM_OPT_NO_ROW_FASTPATH : 0);
+                lbm_domain* D = nullptr;
+                int rc = lbm_domain_create(ctx, &d, &D);
+                if (rc) return rc;
+                double ms = 0.0;
+                rc = lbm_step(D, 64);
+                if (!rc) rc = lbm_step_timed(D, steps, &ms);
+                const int64_t n = D->n;
+                lbm_domain_destroy(D);
+                if (rc) return rc;
+                const double mlups = ms > 0 ? (double)n * steps / (ms * 1e-3) / 1e6 : 0.0;
+                out->ntried++;
+                if (mlups > best) {
+                    best = mlups;
+                    out->scheme = sch;
+                    out->fast_axis = ax;
+                    out->options = d.options;
+                    out->mlups = mlups;
+                }
+            }
+    if (cache_path) {
+        if (FILE* fp = std::fopen(cache_path, "a")) {
+            std::fprintf(fp, "%llx %d %d %d %.6f\n", (unsigned long long)key, out->scheme, out->fast_axis,
+                         out->options, out->mlups);
+            std::fclose(fp);
+        }
+    }
+    return LBM_OK;
+}

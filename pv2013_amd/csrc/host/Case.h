@@ -1,0 +1,37 @@
+// One run described by a configuration file (see LBM.cpp for the keys).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "lbm_capi.h"
+
+struct WaveSpec {
+    int slot = 0;
+    int kind = LBM_WAVE_NONE;
+    double period = 1.0, dt = 1.0;
+    std::vector<double> a, phi;
+};
+
+struct CaseConfig {
+    std::string stl_file;
+    double spacing = 0.1;                 // Input.h:9
+    bool explicit_box = false;
+    double box_origin[3] = {0, 0, 0};
+    int box_dims[3] = {0, 0, 0};
+    std::vector<std::string> plane_names;  // priority order
+    std::vector<lbm_plane_desc> planes;
+    std::vector<lbm_bc_desc> bcs;           // bcs[p] drives plane p (flag 3 + p)
+    std::vector<WaveSpec> waves;
+    int q = 19;
+    int storage = LBM_STORE_F64;
+    int scheme = LBM_SCHEME_TWO_LATTICE;
+    int quirks = 1;
+    double tau = 0.66;                      // LBM.cpp:59
+    long long steps = 20000;                // Simulation.cpp:908
+    int output_every = 100;                 // Simulation.cpp:920
+    std::string output_dir = ".";
+    int device = 0;
+};
+
+bool load_case(const std::string& path, CaseConfig& out, std::string& err);

@@ -1,0 +1,26 @@
+// SubDomain (Subdomain.h:68 interface): the fluid-node lattice of a Geometry,
+// held on the GPU (lbm_domain_create).  One GPU holds the whole domain, so the
+// reference's cartesian decomposition and halo layer have no counterpart.
+#pragma once
+
+#include <vector>
+
+#include "ComplexGeometry.h"
+
+class SubDomain {
+public:
+    SubDomain(Geometry& geometry, const CaseConfig& cfg, lbm_ctx* ctx);
+    ~SubDomain();
+    SubDomain(const SubDomain&) = delete;
+    SubDomain& operator=(const SubDomain&) = delete;
+
+    Geometry& Geometry_;
+    lbm_domain* dom = nullptr;
+    int my_Nx_ = 0, my_Ny_ = 0, my_Nz_ = 0;
+    int64_t fluid_nodes = 0, boundary_records = 0;
+    // i-j-k ordered wall sites (i outermost), the WallBoundaryNode_ order of
+    // Subdomain.cpp:464-469
+    std::vector<int64_t> wall_sites;
+    bool ok = true;
+    std::string error;
+};

@@ -412,29 +412,42 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
         // Wavefront index, made wave-uniform so the loads below that depend
         // only on it are scalar loads.
         const int wv = __builtin_amdgcn_readfirstlane((int)(n >> 6));
-        const bool reg = a.wreg && ((a.wreg[wv >> 5] >> (wv & 31)) & 1u);
-        int32_t p = 0;
-        uint32_t miss = 0;
+        bool reg = a.wreg && ((a.wreg[wv >> 5] >> (wv & 31)) & 1u);
+        int32_t b_src[Q], b_dst[Q];
         if (reg) {
             // All 64 lanes are interior nodes of one contiguous row, so for
             // every direction their sources (and destinations) are one
-            // contiguous run of node ids.
+            // contiguous run of node ids.  The wave-uniform bases are checked
+            // to lie in [0, n - 64] before any lane uses them.
             const int32_t p0 = a.pnode[(int64_t)wv << 6];
+            bool ok = ((int64_t)wv << 6) + 63 < N;
+#pragma unroll
+            for (int d = 0; d < Q; ++d) {
+                b_src[d] = (d == L::rest || SCH == SCH_AA_LOCAL) ? (wv << 6) : a.map[p0 - a.off[d]];
+                b_dst[d] = d == L::rest ? (wv << 6) : (SCH == SCH_AA_GATHER ? a.map[p0 + a.off[d]] : 0);
+                ok = ok && b_src[d] >= 0 && (int64_t)b_src[d] + 63 < N && b_dst[d] >= 0 &&
+                     (int64_t)b_dst[d] + 63 < N;
+            }
+            reg = ok;
+        }
+        int32_t p = 0;
+        uint32_t miss = 0;
+        if (reg) {
             const int lane = (int)(n & 63);
 #pragma unroll
             for (int d = 0; d < Q; ++d) {
                 if (SCH == SCH_AA_LOCAL || d == L::rest) {
-                    f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_LOCAL ? d : d) * N + n);
+                    f[d] = load_pop<P>(gin, (int64_t)d * N + n);
                     dst[d] = (int32_t)n;
                 } else {
-                    const int64_t src = (int64_t)a.map[p0 - a.off[d]] + lane;
+                    const int64_t src = (int64_t)b_src[d] + lane;
                     f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
-                    if (SCH == SCH_AA_GATHER) dst[d] = a.map[p0 + a.off[d]] + lane;
+                    if (SCH == SCH_AA_GATHER) dst[d] = b_dst[d] + lane;
                 }
             }
             if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
         } else {
-            p = a.pnode[n];
+            if (SCH != SCH_AA_LOCAL) p = a.pnode[n];
 #pragma unroll
             for (int d = 0; d < Q; ++d) {
                 if (SCH == SCH_AA_LOCAL || d == L::rest) {

@@ -170,6 +170,37 @@ def y_bifurcation(seed, dims=(256, 256, 512), parent_r=40.0, daughter_r=30.0, an
     return tri, info
 
 
+def bent_pipe(length, radius, curvature_radius=None, h=None, pad=3):
+    """Pipe along z of the given length whose centreline bends in x as
+    x_c(z) = A (1 - cos(2 pi z / L)) with peak curvature 1/curvature_radius
+    (A = L^2 / (4 pi^2 R_c)); both ends are straight and axis-aligned so the
+    inlet (z = 1) and outlet (z = nz - 2) are lattice planes.  No bend when
+    curvature_radius is None.  Returns (triangles, info) in lattice units."""
+    L = float(length)
+    amp = 0.0 if not curvature_radius else L * L / (4 * np.pi ** 2 * curvature_radius)
+    nx = int(math.ceil(2 * radius + 2 * amp)) + 2 * pad
+    ny = int(math.ceil(2 * radius)) + 2 * pad
+    nz = int(length) + 2
+    cx0 = pad + radius
+    cy = (ny - 1) / 2.0
+    z0, z1 = 0.5, nz - 1.5
+    k = 2 * np.pi / (z1 - z0)
+
+    def sdf(X, Y, Z):
+        xc = cx0 + amp * (1 - np.cos(k * (Z - z0)))
+        slope = amp * k * np.sin(k * (Z - z0))
+        cosang = 1.0 / np.sqrt(1.0 + slope * slope)
+        d = _tube_dist(X - xc, Y - cy, radius, None, Z, cosang)
+        return np.maximum(d, np.maximum(z0 - Z, Z - z1))
+
+    if h is None:
+        h = min(1.0, max(0.5, radius / 8.0))
+    tri = marching_tets(sdf, (0.0, 0.0, 0.0), (nx - 1.0, ny - 1.0, nz - 1.0), h)
+    info = {"dims": (nx, ny, nz), "inlet_center": (cx0, cy), "radius": radius, "inlet_z": 1,
+            "outlet_z": nz - 2, "bend": amp}
+    return tri, info
+
+
 def fourier_waveform(seed, nk=4, amp=0.5):
     """Seeded a_k, phi_k for u(t) = U (1 + sum a_k sin(2 pi k t / T + phi_k))."""
     rng = np.random.default_rng(seed + 7919)

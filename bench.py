@@ -30,38 +30,9 @@ SEED = 2013
 PERIOD = 4000.0
 
 
-def c2_case(P, ctx):
-    import numpy as np
-
-    from pv2013_amd.geometry import fourier_waveform, y_bifurcation
-    dims = (256, 256, 512)
-    tri, info = y_bifurcation(SEED, dims)
-    grid = P.make_grid((0.0, 0.0, 0.0), 1.0, dims, (0.0, dims[0] - 1.0, 0.0, dims[1] - 1.0, 0.0, dims[2] - 1.0))
-    flags, vox_ms = ctx.voxelize(tri, grid)
-    cx, cy = info["center"]
-    planes = [P.make_plane(2, "1"),
-              P.make_plane(2, str(info["outlet_z"]), ranges=((0, -1.0, cx),)),
-              P.make_plane(2, str(info["outlet_z"]), ranges=((0, cx, dims[0] + 1.0),))]
-    flags, cls_ms = ctx.classify(grid, flags, planes)
-    bcs = [dict(kind=P.BC_VELOCITY, axis=2, sigma=1, u=(0.0, 0.0, 0.03), profile=P.PROFILE_PARABOLIC,
-                center=(cx, cy), radius=info["parent_r"], waveform=0),
-           dict(kind=P.BC_PRESSURE, axis=2, sigma=-1, rho=1.0),
-           dict(kind=P.BC_PRESSURE, axis=2, sigma=-1, rho=1.0)]
-    a, phi = fourier_waveform(SEED)
-    return {"dims": dims, "flags": flags, "bcs": bcs, "wave": (a, phi), "ntri": len(tri),
-            "vox_ms": vox_ms, "cls_ms": cls_ms, "nsites": int(np.prod(dims))}
-
-
-def make_domain(P, ctx, case, storage):
-    dom = P.Domain(ctx, case["dims"], case["flags"], TAU, [P.make_bc(**b) for b in case["bcs"]],
-                   q=19, storage=storage)
-    a, phi = case["wave"]
-    dom.set_waveform(0, P.WAVE_FOURIER, PERIOD, a, phi)
-    return dom
-
-
-def run(P, ctx, case, storage, steps, warmup):
-    dom = make_domain(P, ctx, case, storage)
+def run(P, ctx, case, storage, steps, warmup, scheme=0):
+    from pv2013_amd.cases import make_domain
+    dom = make_domain(ctx, case, storage=storage, scheme=scheme)
     n = dom.info()["nodes"]
     dom.step(warmup)
     dom.sync()
@@ -73,6 +44,28 @@ def run(P, ctx, case, storage, steps, warmup):
     return {"nodes": n, "ms": ms, "wall_s": wall, "mass": mass}
 
 
+def accuracy(P, ctx, case, storages):
+    """The configuration's protocol: 8000 steps from rest in each storage;
+    velocity, pressure (rho/3) and wall-shear-stress errors against FP64."""
+    from pv2013_amd.cases import errors, make_domain
+    out = {}
+    ref = None
+    for name, st in (("f64", P.STORE_F64),) + tuple(storages):
+        dom = make_domain(ctx, case, storage=st)
+        dom.step(case["steps"])
+        rho, u = dom.macros()
+        wss = dom.wss()
+        dom.close()
+        if ref is None:
+            ref = (rho, u, wss)
+            fl = case["flags"] > 0
+            wl = case["flags"] == 2
+            continue
+        out[name] = {"velocity": errors(ref[1], u, fl), "pressure": errors(ref[0] / 3.0, rho / 3.0, fl),
+                     "wss": errors(ref[2], wss, wl), "steps": case["steps"]}
+    return out
+
+
 def cpu_baseline(case, budget_s=12.0):
     """The FP64 oracle (a plain-C restatement of the reference step), one
     thread, on the same domain for as many steps as fit in about budget_s."""
@@ -80,8 +73,8 @@ def cpu_baseline(case, budget_s=12.0):
     import pyoracle as O
     bcs = [O.make_bc(**b) for b in case["bcs"]]
     sim = O.Sim(19, case["dims"], case["flags"], TAU, bcs, quirks=1)
-    a, phi = case["wave"]
-    sim.set_waveform(0, 1, PERIOD, a, phi)
+    kind, period, a, phi = case["waves"][0]
+    sim.set_waveform(0, kind, period, a, phi)
     t0 = time.perf_counter()
     sim.step(1)
     one = time.perf_counter() - t0
@@ -110,7 +103,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--storage", default="f32", choices=["f32", "f64", "f16s"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 / FP16 comparison runs")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP64 / FP16 comparison runs and the accuracy protocol")
+    ap.add_argument("--scheme", default="two", choices=["two", "aa"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the whole domain runs on a single MI355X "
@@ -120,11 +115,14 @@ def main():
     storage = {"f32": P.STORE_F32, "f64": P.STORE_F64, "f16s": P.STORE_F16S}[args.storage]
     sbytes = {"f32": 4, "f64": 8, "f16s": 2}[args.storage]
     ctx = P.Context(0)
+    from pv2013_amd.cases import c2
     print("bench: building the 256x256x512 Y-bifurcation case", file=sys.stderr, flush=True)
-    case = c2_case(P, ctx)
+    case = c2(ctx, seed=SEED, tau=TAU, period=PERIOD)
+    case["nsites"] = 256 * 256 * 512
     print("bench: timing %d steps (%s) after %d warm-up" % (args.steps, args.storage, args.warmup),
           file=sys.stderr, flush=True)
-    r = run(P, ctx, case, storage, args.steps, args.warmup)
+    scheme = P.SCHEME_AA if args.scheme == "aa" else P.SCHEME_TWO_LATTICE
+    r = run(P, ctx, case, storage, args.steps, args.warmup, scheme=scheme)
     n = r["nodes"]
     ms_step = r["ms"] / args.steps
     mlups = n * args.steps / (r["ms"] * 1e-3) / 1e6
@@ -132,7 +130,7 @@ def main():
     achieved = bytes_per_launch / (ms_step * 1e-3) / 1e9
     extra = {"fluid_nodes": n, "box_sites": case["nsites"], "box_site_mlups": case["nsites"] / n * mlups,
              "wall_s_timed": r["wall_s"], "mass_after": r["mass"], "facets": case["ntri"],
-             "voxelize_ms": case["vox_ms"], "classify_ms": case["cls_ms"],
+             "voxelize_ms": case["vox_ms"], "classify_ms": case["cls_ms"], "scheme": args.scheme,
              "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3)}
     if not args.no_compare:
         for name, st in (("f64", P.STORE_F64), ("f16s", P.STORE_F16S)):
@@ -143,6 +141,8 @@ def main():
             extra["mlups_" + name] = rr["nodes"] * min(args.steps, 1000) / (rr["ms"] * 1e-3) / 1e6
         if "mlups_f64" in extra:
             extra["speedup_vs_f64"] = mlups / extra["mlups_f64"]
+        print("bench: accuracy protocol (%d steps per storage)" % case["steps"], file=sys.stderr, flush=True)
+        extra["errors_vs_f64"] = accuracy(P, ctx, case, (("f32", P.STORE_F32), ("f16s", P.STORE_F16S)))
     cpu = None
     if not args.no_cpu:
         print("bench: CPU oracle sample", file=sys.stderr, flush=True)
@@ -158,13 +158,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f32": "f32 (f-w storage), f32 collide", "f64": "f64",
-                  "f16s": "f16 (f-w storage), f32 collide"}[args.storage],
+        "dtype": "f64" if args.storage == "f64" else "f32",
         "data": "synthetic (seeded Y-bifurcation STL, seed %d)" % SEED,
         "config": {"workload": "D3Q19 BGK, 256x256x512 Y-bifurcation, pulsatile Zou-He inlet, "
                                "pressure outlets, bounce-back walls, tau 0.66",
-                   "lattice": "D3Q19", "box": [256, 256, 512], "storage": args.storage,
-                   "scheme": "two-lattice pull", "parallelism": "single GPU"},
+                   "lattice": "D3Q19", "box": [256, 256, 512],
+                   "storage": {"f32": "f32 holding f - w", "f64": "f64",
+                               "f16s": "f16 holding f - w"}[args.storage],
+                   "scheme": "AA" if args.scheme == "aa" else "two-lattice pull",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.storage)},
         "cpu_baseline": cpu,

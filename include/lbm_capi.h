@@ -45,6 +45,8 @@ typedef struct lbm_domain lbm_domain;
  * cartesian communicator set-up (LBM.cpp:25, Subdomain.cpp:67-230): one GPU
  * holds the whole domain. */
 int lbm_ctx_create(int device, lbm_ctx** out);
+/* Also destroys every domain still open on the context (their handles become
+ * invalid). */
 void lbm_ctx_destroy(lbm_ctx* ctx);
 /* Text of the last error on this context (never NULL). */
 const char* lbm_last_error(const lbm_ctx* ctx);

@@ -5,6 +5,7 @@ loudly when the shared library has not been built.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -197,9 +198,13 @@ class Context:
         if rc:
             raise LbmError("lbm_ctx_create(%d) failed (%d)" % (device, rc))
         self.h = h
+        self.domains = weakref.WeakSet()
 
     def close(self):
+        """Destroys the context and every domain still open on it."""
         if getattr(self, "h", None):
+            for d in list(self.domains):
+                d.h = None  # freed by lbm_ctx_destroy below
             lib().lbm_ctx_destroy(self.h)
             self.h = None
 
@@ -261,6 +266,7 @@ class Domain:
         self.dims = tuple(int(d) for d in dims)
         self.nsites = self.dims[0] * self.dims[1] * self.dims[2]
         self.nbc = len(bcs)
+        ctx.domains.add(self)
         if _handle is not None:
             self.h = _handle
             return
@@ -272,7 +278,8 @@ class Domain:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().lbm_domain_destroy(self.h)
+            if getattr(self.ctx, "h", None):
+                lib().lbm_domain_destroy(self.h)
             self.h = None
 
     def __del__(self):

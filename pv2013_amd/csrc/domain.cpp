@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -31,6 +32,7 @@ struct lbm_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
+    std::set<lbm_domain*> domains;  // live domains; destroyed with the context
 };
 
 int lbm_ctx_fail(lbm_ctx* ctx, int code, const std::string& msg) {
@@ -67,10 +69,13 @@ extern "C" int lbm_ctx_create(int device, lbm_ctx** out) {
     return LBM_OK;
 }
 
+extern "C" void lbm_domain_destroy(lbm_domain* D);
+
 extern "C" void lbm_ctx_destroy(lbm_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    while (!ctx->domains.empty()) lbm_domain_destroy(*ctx->domains.begin());
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -517,6 +522,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(hipStreamSynchronize(ctx->stream));
 #undef DCHECK
     D->state = 0;
+    ctx->domains.insert(D);
     *out = D;
     return LBM_OK;
 }
@@ -525,6 +531,7 @@ extern "C" void lbm_domain_destroy(lbm_domain* D) {
     if (!D) return;
     (void)hipSetDevice(D->ctx->device);
     (void)hipStreamSynchronize(D->ctx->stream);
+    D->ctx->domains.erase(D);
     delete D;
 }
 

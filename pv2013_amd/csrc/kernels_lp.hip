@@ -1,5 +1,5 @@
 // FP32 / FP16 shifted-storage kernels (populations stored as f - w, collision
-// in FP32).  Built with FMA contraction allowed.
+// in FP32).
 #include "step_kernels.h"
 
 using namespace lbm;

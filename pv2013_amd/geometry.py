@@ -201,6 +201,122 @@ def bent_pipe(length, radius, curvature_radius=None, h=None, pad=3):
     return tri, info
 
 
+def _catmull_rom(P, samples_per_seg=200):
+    """Centripetal-free uniform Catmull-Rom spline through control points P."""
+    P = np.asarray(P, float)
+    ext = np.vstack([2 * P[0] - P[1], P, 2 * P[-1] - P[-2]])
+    out = []
+    t = np.linspace(0, 1, samples_per_seg, endpoint=False)[:, None]
+    for i in range(1, len(ext) - 2):
+        p0, p1, p2, p3 = ext[i - 1], ext[i], ext[i + 1], ext[i + 2]
+        out.append(0.5 * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t * t
+                          + (-p0 + 3 * p1 - 3 * p2 + p3) * t * t * t))
+    out.append(P[-1][None, :])
+    return np.vstack(out)
+
+
+def tortuous_tube(seed, n=512, r_min=10.0, r_max=60.0, ncontrol=7, h=2.2):
+    """Closed tube around a seeded random cubic (Catmull-Rom) centreline inside
+    an n^3 box, radius varying smoothly in [r_min, r_max]; capped where the
+    centreline ends.  About 500k facets at the default h for n = 512."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    m = r_max + 8.0
+    ctrl = m + (n - 1 - 2 * m) * rng.random((ncontrol, 3))
+    ctrl[0, 2], ctrl[-1, 2] = m, n - 1 - m
+    ctrl = ctrl[np.argsort(ctrl[:, 2])]
+    cl = _catmull_rom(ctrl)
+    cl = np.clip(cl, m, n - 1 - m)
+    seg = np.linalg.norm(np.diff(cl, axis=0), axis=1)
+    s = np.concatenate([[0.0], np.cumsum(seg)])
+    smax = s[-1]
+    # radius along the arc length: smooth random mix of a few cosines
+    ks = rng.integers(1, 5, 3)
+    ph = 2 * np.pi * rng.random(3)
+    wave = sum(np.cos(2 * np.pi * k * s / smax + p) for k, p in zip(ks, ph)) / 3.0
+    rad = r_min + (r_max - r_min) * 0.5 * (1 + wave)
+    # resample densely so the nearest sample approximates the curve distance
+    tree = cKDTree(cl)
+    tangent_end = [cl[1] - cl[0], cl[-1] - cl[-2]]
+    tangent_end = [t / np.linalg.norm(t) for t in tangent_end]
+
+    def sdf(X, Y, Z):
+        pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+        dist, idx = tree.query(pts)
+        d = dist - rad[idx]
+        # flat caps at both ends of the centreline
+        c0 = ((pts - cl[0]) * tangent_end[0]).sum(1)
+        c1 = ((pts - cl[-1]) * tangent_end[1]).sum(1)
+        d = np.maximum(d, np.maximum(-c0, c1))
+        return d.reshape(X.shape)
+
+    tri = marching_tets(sdf, (0.0, 0.0, 0.0), (n - 1.0, n - 1.0, n - 1.0), h, slab=4)
+    return tri, {"dims": (n, n, n), "centreline_length": smax}
+
+
+def vessel_tree(seed, dims=(768, 384, 384), r_root=60.0, r_leaf=15.0, levels=3, amp=1.5, nmodes=8, h=3.0):
+    """Seeded multi-branch tree along +x: a root tube from the inlet plane
+    x = 0.5 that bifurcates `levels` times (alternately in y and z), radii
+    shrinking geometrically from r_root to r_leaf, every leaf running parallel
+    to x to the outlet plane x = nx - 1.5; wall perturbation as in the
+    Y-bifurcation.  Returns (triangles, info)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    cy, cz = (ny - 1) / 2.0, (nz - 1) / 2.0
+    x0, x1 = 0.5, nx - 1.5
+    seglen = (x1 - x0) / (levels + 1)
+    ratio = (r_leaf / r_root) ** (1.0 / levels)
+    # branch k at level l: lateral offset of its centre, reached with a smooth step
+    spread_y, spread_z = (ny - 2 * r_root) / 4.0, (nz - 2 * r_root) / 4.0
+    frontier = [(cy, cz)]
+    tubes = []
+    for lev in range(levels + 1):
+        r = r_root * ratio ** lev
+        xs = x0 + lev * seglen
+        nxt = []
+        for (py, pz) in frontier:
+            tubes.append((xs, r, py, pz))
+            if lev < levels:
+                dy = spread_y / (2 ** (lev // 2)) if lev % 2 == 0 else 0.0
+                dz = spread_z / (2 ** (lev // 2)) if lev % 2 == 1 else 0.0
+                nxt += [(py - dy, pz - dz), (py + dy, pz + dz)]
+        frontier = nxt
+    modes = [WallModes(rng, nmodes, amp, nx) for _ in tubes]
+    # each tube: from x = xs (where it leaves its parent's end position) to x1,
+    # its centre moving smoothly from the parent's centre to its own
+    tube_par = []
+    lev_of = []
+    k = 0
+    for lev in range(levels + 1):
+        for j in range(2 ** lev):
+            xs, r, py, pz = tubes[k]
+            if lev == 0:
+                tube_par.append((cy, cz))
+            else:
+                tube_par.append(tubes[(2 ** (lev - 1) - 1) + j // 2][2:4])
+            lev_of.append(lev)
+            k += 1
+
+    def sdf(X, Y, Z):
+        d = np.full(X.shape, np.inf)
+        for (xs, r, py, pz), (qy, qz), md, lev in zip(tubes, tube_par, modes, lev_of):
+            t = np.clip((X - xs) / (0.6 * seglen), 0.0, 1.0)
+            sm = t * t * (3 - 2 * t)
+            yc = qy + (py - qy) * sm
+            zc = qz + (pz - qz) * sm
+            dt = _tube_dist(Y - yc, Z - zc, r, md, X)
+            dt = np.maximum(dt, (xs - r * 0.5) - X)
+            if lev < levels:  # a parent ends inside its children's start
+                dt = np.maximum(dt, X - (xs + seglen + 0.5 * r))
+            d = np.minimum(d, dt)
+        return np.maximum(d, np.maximum(x0 - X, X - x1))
+
+    tri = marching_tets(sdf, (0.0, 0.0, 0.0), (nx - 1.0, ny - 1.0, nz - 1.0), h, slab=6)
+    info = {"dims": dims, "inlet_x": 1, "outlet_x": nx - 2, "root_center": (cy, cz), "r_root": r_root,
+            "leaves": 2 ** levels}
+    return tri, info
+
+
 def fourier_waveform(seed, nk=4, amp=0.5):
     """Seeded a_k, phi_k for u(t) = U (1 + sum a_k sin(2 pi k t / T + phi_k))."""
     rng = np.random.default_rng(seed + 7919)

@@ -33,7 +33,8 @@ PERIOD = 4000.0
 def run(P, ctx, case, storage, steps, warmup, scheme=0):
     from pv2013_amd.cases import make_domain
     dom = make_domain(ctx, case, storage=storage, scheme=scheme)
-    n = dom.info()["nodes"]
+    st = dom.stats()
+    n = st["nodes"]
     dom.step(warmup)
     dom.sync()
     t0 = time.perf_counter()
@@ -41,7 +42,7 @@ def run(P, ctx, case, storage, steps, warmup, scheme=0):
     wall = time.perf_counter() - t0
     mass, _ = dom.reduce()
     dom.close()
-    return {"nodes": n, "ms": ms, "wall_s": wall, "mass": mass}
+    return {"nodes": n, "ms": ms, "wall_s": wall, "mass": mass, "stats": st}
 
 
 def accuracy(P, ctx, case, storages):
@@ -131,7 +132,9 @@ def main():
     extra = {"fluid_nodes": n, "box_sites": case["nsites"], "box_site_mlups": case["nsites"] / n * mlups,
              "wall_s_timed": r["wall_s"], "mass_after": r["mass"], "facets": case["ntri"],
              "voxelize_ms": case["vox_ms"], "classify_ms": case["cls_ms"], "scheme": args.scheme,
-             "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3)}
+             "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3),
+             "regular_wave_fraction": r["stats"]["regular_waves"] / max(1, r["stats"]["waves"]),
+             "boundary_records": r["stats"]["boundary_records"], "node_order_fast_axis": r["stats"]["fast_axis"]}
     if not args.no_compare:
         for name, st in (("f64", P.STORE_F64), ("f16s", P.STORE_F16S)):
             if name == args.storage:

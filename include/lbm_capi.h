@@ -139,6 +139,13 @@ typedef struct {
 /* lbm_domain_desc.options */
 #define LBM_OPT_NO_ROW_FASTPATH 1  /* address every lane through the site map */
 #define LBM_OPT_NO_GRAPH 2         /* launch every step eagerly instead of replaying a hipGraph */
+/* Periodic axes: the halo of the site map holds the wrapped node ids (a ghost
+ * fill done once at build time), so streaming wraps at no cost per step.  The
+ * reference's communicator is non-periodic (Subdomain.cpp:70); this replaces
+ * its disabled halo exchange (Simulation.cpp:202-872) on one GPU. */
+#define LBM_OPT_PERIODIC_X 4
+#define LBM_OPT_PERIODIC_Y 8
+#define LBM_OPT_PERIODIC_Z 16
 
 /* SubDomain(Geometry&) (Subdomain.cpp:3-33: CreateNodes, Connectivity,
  * Initialize, GetFluidHaloLat) + Simulation's constructor set-up
@@ -148,6 +155,15 @@ int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_domain** ou
 void lbm_domain_destroy(lbm_domain* dom);
 int lbm_domain_info(const lbm_domain* dom, int64_t* n_nodes, int64_t* n_boundary,
                     int64_t* device_bytes);
+
+typedef struct {
+    int64_t nodes, boundary_records, wall_nodes, bc_nodes;
+    int64_t waves, regular_waves;  /* 64-node wavefronts; those on the row fast path */
+    int64_t device_bytes;
+    int32_t fast_axis;             /* node-order fastest axis */
+    int32_t scheme;
+} lbm_domain_stats;
+int lbm_domain_get_stats(const lbm_domain* dom, lbm_domain_stats* out);
 
 /* Pulsatile inlet waveform for slot (Simulation.cpp:875-900, pulsatile.h).
  * FOURIER: W(t) = 1 + sum_k a_k sin(2 pi k t / period + phi_k), t in steps.

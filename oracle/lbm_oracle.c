@@ -500,6 +500,14 @@ int64_t orc_sim_iter(const orc_sim* s) { return s->iter; }
 
 orc_sim* orc_sim_create(int q, const int32_t dims[3], const uint8_t* flags, double tau,
                         const orc_bc* bcs, int nbc, int quirks) {
+    return orc_sim_create_periodic(q, dims, flags, tau, bcs, nbc, quirks, 0);
+}
+
+/* Periodic axes have no counterpart in the reference, whose cartesian
+ * communicator is non-periodic (Subdomain.cpp:70); a periodic neighbour is the
+ * node at the wrapped coordinate. */
+orc_sim* orc_sim_create_periodic(int q, const int32_t dims[3], const uint8_t* flags, double tau,
+                                 const orc_bc* bcs, int nbc, int quirks, int periodic) {
     if (q != 19 && q != 27) return NULL;
     orc_sim* s = (orc_sim*)calloc(1, sizeof *s);
     s->q = q;
@@ -540,6 +548,9 @@ orc_sim* orc_sim_create(int q, const int32_t dims[3], const uint8_t* flags, doub
                 s->type[m] = flags[st];
                 for (int d = 0; d < q; ++d) {
                     int a = i + c[d][0], b = j + c[d][1], e = k + c[d][2];
+                    if (periodic & 1) a = (a + nx) % nx;
+                    if (periodic & 2) b = (b + ny) % ny;
+                    if (periodic & 4) e = (e + nz) % nz;
                     int64_t nbn = -1;
                     if (a >= 0 && a < nx && b >= 0 && b < ny && e >= 0 && e < nz)
                         nbn = s->node[a + nx * (b + (int64_t)ny * e)];

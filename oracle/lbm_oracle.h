@@ -82,6 +82,9 @@ typedef struct orc_sim orc_sim;
 
 orc_sim* orc_sim_create(int q, const int32_t dims[3], const uint8_t* flags, double tau,
                         const orc_bc* bcs, int nbc, int quirks);
+/* periodic: bit a makes axis a periodic (neighbours wrap around the box) */
+orc_sim* orc_sim_create_periodic(int q, const int32_t dims[3], const uint8_t* flags, double tau,
+                                 const orc_bc* bcs, int nbc, int quirks, int periodic);
 void orc_sim_destroy(orc_sim* s);
 int orc_sim_set_waveform(orc_sim* s, int slot, int kind, double period, int nk,
                          const double* a, const double* phi, double dt);

@@ -57,6 +57,9 @@ def lib():
         L.orc_sim_create.argtypes = [C.c_int, P(np.int32), P(np.uint8), C.c_double, C.c_void_p,
                                      C.c_int, C.c_int]
         L.orc_sim_create.restype = C.c_void_p
+        L.orc_sim_create_periodic.argtypes = [C.c_int, P(np.int32), P(np.uint8), C.c_double, C.c_void_p,
+                                              C.c_int, C.c_int, C.c_int]
+        L.orc_sim_create_periodic.restype = C.c_void_p
         L.orc_sim_destroy.argtypes = [C.c_void_p]
         L.orc_sim_set_waveform.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
                                            P(np.float64), P(np.float64), C.c_double]
@@ -169,15 +172,15 @@ def make_bc(kind, axis, sigma, rho=1.0, u=(0.0, 0.0, 0.0), profile=0, center=(0.
 class Sim:
     """Reference-semantics FP64 solver over a flag box (x-fastest sites)."""
 
-    def __init__(self, q, dims, flags, tau, bcs=(), quirks=1):
+    def __init__(self, q, dims, flags, tau, bcs=(), quirks=1, periodic=0):
         self.q = q
         self.dims = tuple(int(d) for d in dims)
         self.nsites = self.dims[0] * self.dims[1] * self.dims[2]
         arr = (BC * max(1, len(bcs)))(*bcs)
         self._bcs = arr
-        self.h = lib().orc_sim_create(q, np.array(self.dims, np.int32),
-                                      np.ascontiguousarray(flags, np.uint8), tau,
-                                      C.cast(arr, C.c_void_p), len(bcs), quirks)
+        self.h = lib().orc_sim_create_periodic(q, np.array(self.dims, np.int32),
+                                               np.ascontiguousarray(flags, np.uint8), tau,
+                                               C.cast(arr, C.c_void_p), len(bcs), quirks, periodic)
         if not self.h:
             raise ValueError("orc_sim_create failed")
 

@@ -17,6 +17,7 @@ STORE_F64, STORE_F32, STORE_F16S = 0, 1, 2
 SCHEME_TWO_LATTICE, SCHEME_AA = 0, 1
 OPT_NO_ROW_FASTPATH = 1
 OPT_NO_GRAPH = 2
+OPT_PERIODIC_X, OPT_PERIODIC_Y, OPT_PERIODIC_Z = 4, 8, 16
 BC_VELOCITY, BC_PRESSURE = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 WAVE_NONE, WAVE_FOURIER, WAVE_PV2013 = 0, 1, 2
@@ -29,6 +30,7 @@ EXPORTS = (
     "lbm_domain_destroy", "lbm_domain_info", "lbm_set_waveform", "lbm_step", "lbm_step_timed",
     "lbm_sync", "lbm_iteration", "lbm_get_f", "lbm_set_f", "lbm_get_macros", "lbm_get_wss",
     "lbm_reduce", "lbm_get_macros_box", "lbm_get_wss_box", "lbm_batch_create", "lbm_autotune",
+    "lbm_domain_get_stats",
 )
 
 
@@ -60,6 +62,12 @@ class DomainDesc(C.Structure):
                 ("scheme", C.c_int32), ("quirks", C.c_int32), ("block", C.c_int32),
                 ("tau", C.c_double), ("flags", C.c_void_p), ("bcs", C.c_void_p),
                 ("nbc", C.c_int32), ("fast_axis", C.c_int32), ("options", C.c_int32)]
+
+
+class DomainStats(C.Structure):
+    _fields_ = [("nodes", C.c_int64), ("boundary_records", C.c_int64), ("wall_nodes", C.c_int64),
+                ("bc_nodes", C.c_int64), ("waves", C.c_int64), ("regular_waves", C.c_int64),
+                ("device_bytes", C.c_int64), ("fast_axis", C.c_int32), ("scheme", C.c_int32)]
 
 
 class TuneResult(C.Structure):
@@ -113,6 +121,7 @@ def lib():
         L.lbm_get_macros_box.argtypes = [vp, vp, vp, vp, vp]
         L.lbm_get_wss_box.argtypes = [vp, vp, vp, vp]
         L.lbm_batch_create.argtypes = [vp, vp, i32, i32, P(vp), vp]
+        L.lbm_domain_get_stats.argtypes = [vp, P(DomainStats)]
         L.lbm_autotune.argtypes = [vp, P(DomainDesc), i32, C.c_char_p, P(TuneResult)]
         _lib = L
     return _lib
@@ -289,6 +298,11 @@ class Domain:
         n, nb, by = C.c_int64(), C.c_int64(), C.c_int64()
         lib().lbm_domain_info(self.h, C.byref(n), C.byref(nb), C.byref(by))
         return {"nodes": n.value, "boundary_records": nb.value, "device_bytes": by.value}
+
+    def stats(self):
+        st = DomainStats()
+        self.ctx.check(lib().lbm_domain_get_stats(self.h, C.byref(st)), "lbm_domain_get_stats")
+        return {f: getattr(st, f) for f, _ in DomainStats._fields_}
 
     def set_waveform(self, slot, kind, period, a=(), phi=(), dt=1.0):
         a = np.ascontiguousarray(a if len(a) else [0.0], np.float64)

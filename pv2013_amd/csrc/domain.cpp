@@ -114,7 +114,6 @@ struct lbm_domain {
     FaceTab* d_faces = nullptr;
     WaveTab* d_waves = nullptr;
     int64_t* d_iter = nullptr;
-    uint32_t* d_ticket = nullptr;
     void* g[2] = {nullptr, nullptr};
     int32_t *d_wall = nullptr, *d_bcnode = nullptr, *d_bcface = nullptr;
     double *fpre = nullptr, *rho = nullptr, *u = nullptr, *wss = nullptr, *red = nullptr;
@@ -243,7 +242,6 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.faces = D->d_faces;
     a.waves = D->d_waves;
     a.iter = D->d_iter;
-    a.ticket = D->d_ticket;
     a.omega = 1.0 / D->tau;
     for (int d = 0; d < 27; ++d) a.off[d] = D->off[d];
 }
@@ -254,10 +252,15 @@ static int scheme_for(const lbm_domain* D, int64_t it) {
     return (it & 1) == 0 ? 1 : 2;  // AA: gather on even iterations, local on odd
 }
 
-static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* gout) {
+// it_abs = 1: the launch runs iteration it_off; it_abs = 0: iteration
+// (*d_iter + it_off), used inside graphs.
+static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* gout, int64_t it_off,
+                       int it_abs) {
     lbm_ctx* ctx = D->ctx;
     StepArgs a;
     fill_args(D, a);
+    a.it_off = it_off;
+    a.it_abs = it_abs;
     a.gin = gin;
     a.gout = gout;
     a.fpre = D->fpre;
@@ -370,6 +373,30 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
                 site.push_back(st);
             }
     D->n = (int64_t)site.size();
+    // periodic axes: fill the map's halo with the node ids of the opposite face
+    const int periodic = ((d.options & LBM_OPT_PERIODIC_X) ? 1 : 0) | ((d.options & LBM_OPT_PERIODIC_Y) ? 2 : 0) |
+                         ((d.options & LBM_OPT_PERIODIC_Z) ? 4 : 0);
+    if (periodic) {
+        int pw[3];  // periodic flag per permuted axis
+        for (int k = 0; k < 3; ++k) pw[k] = (periodic >> D->perm[k]) & 1;
+        for (int s2 = 0; s2 < D->pd[2]; ++s2)
+            for (int s1 = 0; s1 < D->pd[1]; ++s1)
+                for (int s0 = 0; s0 < D->pd[0]; ++s0) {
+                    int sc[3] = {s0, s1, s2};
+                    bool halo = false, ok = true;
+                    for (int k = 0; k < 3; ++k) {
+                        if (sc[k] == 0 || sc[k] == D->pd[k] - 1) {
+                            halo = true;
+                            if (!pw[k]) ok = false;
+                            else sc[k] = sc[k] == 0 ? D->pd[k] - 2 : 1;
+                        }
+                    }
+                    if (!halo || !ok) continue;
+                    const int64_t p = s0 + (int64_t)D->pd[0] * (s1 + (int64_t)D->pd[1] * s2);
+                    const int64_t w = sc[0] + (int64_t)D->pd[0] * (sc[1] + (int64_t)D->pd[1] * sc[2]);
+                    map[p] = map[w];
+                }
+    }
     if (D->n >= (int64_t)INT32_MAX / 2) {
         delete D;
         return lbm_ctx_fail(ctx, LBM_ERANGE, "too many fluid nodes");
@@ -399,7 +426,10 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
         uint32_t miss = 0;
         for (int e = 0; e < q; ++e) {
             if (e == rest) continue;
-            const int x = ijk[0] - lat_c(q, e, 0), y = ijk[1] - lat_c(q, e, 1), z = ijk[2] - lat_c(q, e, 2);
+            int x = ijk[0] - lat_c(q, e, 0), y = ijk[1] - lat_c(q, e, 1), z = ijk[2] - lat_c(q, e, 2);
+            if (periodic & 1) x = (x + nx) % nx;
+            if (periodic & 2) y = (y + ny) % ny;
+            if (periodic & 4) z = (z + nz) % nz;
             const bool in = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
             if (!in || !flags[x + (int64_t)nx * (y + (int64_t)ny * z)]) miss |= 1u << e;
         }
@@ -478,7 +508,6 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->d_faces, D->faces.size()));
     DCHECK(D->alloc(&D->d_waves, D->waves.size()));
     DCHECK(D->alloc(&D->d_iter, 1));
-    DCHECK(D->alloc(&D->d_ticket, 1));
     uint8_t *g0 = nullptr, *g1 = nullptr;
     DCHECK(D->alloc(&g0, (size_t)q * D->n * sb));
     if (d.scheme == LBM_SCHEME_TWO_LATTICE) DCHECK(D->alloc(&g1, (size_t)q * D->n * sb));
@@ -509,9 +538,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if ((rc = upload(D, D->d_bcnode, D->bc_nodes.data(), sizeof(int32_t) * D->bc_nodes.size()))) return fail(rc);
     if ((rc = upload(D, D->d_bcface, D->bc_face.data(), sizeof(int32_t) * D->bc_face.size()))) return fail(rc);
     const int64_t zero64 = 0;
-    const uint32_t zero32 = 0;
     if ((rc = upload(D, D->d_iter, &zero64, sizeof zero64))) return fail(rc);
-    if ((rc = upload(D, D->d_ticket, &zero32, sizeof zero32))) return fail(rc);
     // Node.cpp:30-37: f = w everywhere, so the stale populations hold w
     std::vector<double> stale((size_t)q * std::max<int64_t>(D->nrec, 1), 0.0);
     for (int e = 0; e < q; ++e)
@@ -540,6 +567,21 @@ extern "C" int lbm_domain_info(const lbm_domain* D, int64_t* n_nodes, int64_t* n
     if (n_nodes) *n_nodes = D->n;
     if (n_boundary) *n_boundary = D->nrec;
     if (device_bytes) *device_bytes = D->device_bytes;
+    return LBM_OK;
+}
+
+extern "C" int lbm_domain_get_stats(const lbm_domain* D, lbm_domain_stats* o) {
+    if (!D || !o) return LBM_EINVAL;
+    std::memset(o, 0, sizeof *o);
+    o->nodes = D->n;
+    o->boundary_records = D->nrec;
+    o->wall_nodes = (int64_t)D->wall_nodes.size();
+    o->bc_nodes = (int64_t)D->bc_nodes.size();
+    o->waves = (D->n + 63) / 64;
+    o->regular_waves = D->nreg_waves;
+    o->device_bytes = D->device_bytes;
+    o->fast_axis = D->perm[0];
+    o->scheme = D->scheme;
     return LBM_OK;
 }
 
@@ -579,7 +621,9 @@ static int build_graph(lbm_domain* D, int c) {
     CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = LBM_OK;
     for (int i = 0; i < lbm_domain::GRAPH_STEPS && rc == LBM_OK; ++i)
-        rc = launch_step(D, LBM_MODE_STEP, scheme_for(D, i), D->g[(c + i) & 1], D->g[(c + i + 1) & 1]);
+        rc = launch_step(D, LBM_MODE_STEP, scheme_for(D, i), D->g[(c + i) & 1], D->g[(c + i + 1) & 1], i, 0);
+    if (rc == LBM_OK && launch_advance_iter(D->d_iter, lbm_domain::GRAPH_STEPS, ctx->stream) != hipSuccess)
+        rc = LBM_EIO;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) {
@@ -593,7 +637,7 @@ static int build_graph(lbm_domain* D, int c) {
 }
 
 static int eager_step(lbm_domain* D, int mode) {
-    int rc = launch_step(D, mode, scheme_for(D, D->iter), D->g[D->cur], D->g[D->cur ^ 1]);
+    int rc = launch_step(D, mode, scheme_for(D, D->iter), D->g[D->cur], D->g[D->cur ^ 1], D->iter, 1);
     if (rc) return rc;
     if (D->scheme == LBM_SCHEME_TWO_LATTICE) D->cur ^= 1;
     D->iter++;
@@ -615,6 +659,7 @@ static int step_n(lbm_domain* D, int64_t nsteps) {
             body--;
         }
         const int c = aa ? 0 : D->cur;
+        if (body >= G) CHECK(launch_set_iter(D->d_iter, D->iter, ctx->stream));  // graph base
         while (body >= G) {
             if (!D->exec[c]) {
                 int rc = build_graph(D, c);
@@ -711,7 +756,7 @@ static int ensure_pre(lbm_domain* D) {
     const unsigned grid = (unsigned)((D->n + 255) / 256);
     if (D->state == 2) {
         if (D->scheme == LBM_SCHEME_AA) return lbm_ctx_fail(ctx, LBM_EIO, "AA state snapshot missing");
-        rc = launch_step(D, LBM_MODE_PRE, 0, D->g[D->cur ^ 1], nullptr);
+        rc = launch_step(D, LBM_MODE_PRE, 0, D->g[D->cur ^ 1], nullptr, D->iter - 1, 1);
         if (rc) return rc;
     } else {
         if (D->state == 0) hipLaunchKernelGGL(k_fill_rest, dim3(grid), dim3(256), 0, ctx->stream, D->fpre, D->n, D->q);

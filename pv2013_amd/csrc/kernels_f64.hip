@@ -6,6 +6,23 @@
 
 using namespace lbm;
 
+// One-thread kernels that move the device iteration base (graph chunks read
+// it; kernels keep it in device memory so replays need no argument updates).
+__global__ void k_advance_iter(int64_t* iter, int64_t by) { *iter = *iter + by; }
+__global__ void k_set_iter(int64_t* iter, int64_t value) { *iter = value; }
+
+hipError_t launch_advance_iter(int64_t* iter, int64_t by, hipStream_t s) {
+    if (!iter) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_advance_iter, dim3(1), dim3(1), 0, s, iter, by);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s) {
+    if (!iter) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_set_iter, dim3(1), dim3(1), 0, s, iter, value);
+    return hipGetLastError();
+}
+
 hipError_t launch_step_f64(int mode, int sch, const StepArgs& a, hipStream_t s) {
     if (a.q == 19) return launch_step_t<19, PrecF64>(mode, sch, a, s);
     return launch_step_t<27, PrecF64>(mode, sch, a, s);

@@ -49,8 +49,9 @@ struct StepArgs {
     const void* bstale;     // [e][r] stale populations (double, or float deviations)
     const FaceTab* faces;
     const WaveTab* waves;
-    int64_t* iter;          // device iteration counter
-    uint32_t* ticket;       // blocks finished in the current launch
+    int64_t* iter;          // device iteration base (graph replays)
+    int64_t it_off;         // iteration of this launch: it_off, or *iter + it_off
+    int32_t it_abs;         //   when it_abs is 0 (graph nodes read the base)
     const void* gin;        // [d][n] post-collision populations
     void* gout;
     double omega;
@@ -84,6 +85,8 @@ struct WssArgs {
 enum { LBM_MODE_STEP = 0, LBM_MODE_PRE = 1, LBM_MODE_SNAP = 2 };
 
 // kernels_f64.hip
+hipError_t launch_advance_iter(int64_t* iter, int64_t by, hipStream_t s);
+hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s);
 // sch: 0 two-lattice pull, 1 AA gather step, 2 AA local step
 hipError_t launch_step_f64(int mode, int sch, const StepArgs& a, hipStream_t s);
 hipError_t launch_init_f64(int q, bool swap, const InitArgs& a, hipStream_t s);

@@ -41,20 +41,6 @@ __device__ __forceinline__ int64_t logical_block() {
     return (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
-// Advance the device iteration counter once every block of the launch is done.
-__device__ __forceinline__ void finish_launch(int64_t* iter, uint32_t* ticket) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            *iter = *iter + 1;
-            *ticket = 0u;
-            __threadfence();
-        }
-    }
-}
-
 // ------------------------------------------------------------- waveforms (FP64)
 
 // pulsatile.h:12-28
@@ -488,7 +474,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
                     if (F.kind == LBM_BC_VELOCITY) {
                         double w = 1.0;
                         if (F.waveform >= 0) {
-                            const int64_t it = *a.iter - (MODE == LBM_MODE_PRE ? 1 : 0);
+                            const int64_t it = a.it_abs ? a.it_off : *a.iter + a.it_off;
                             w = wave_value(a.waves[F.waveform], it);
                         }
                         const double fac = a.bprof[r] * w;
@@ -522,7 +508,6 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
             }
         }
     }
-    if (MODE != LBM_MODE_PRE) finish_launch(a.iter, a.ticket);
 }
 
 // Initial lattice from a given pre-collision state (or f = w when fpre is
@@ -557,7 +542,7 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a) {
 inline bool step_args_ok(int mode, int sch, const StepArgs& a) {
     if (!a.pnode || !a.map || !a.bbits || !a.bpre || !a.gin || !a.faces || !a.waves || !a.iter) return false;
     if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale || !a.bmiss)) return false;
-    if (mode != LBM_MODE_PRE && (!a.gout || !a.ticket)) return false;
+    if (mode != LBM_MODE_PRE && !a.gout) return false;
     if (mode != LBM_MODE_STEP && (!a.fpre || !a.rho || !a.u)) return false;
     if (mode == LBM_MODE_PRE && sch != SCH_TWO) return false;
     return sch >= SCH_TWO && sch <= SCH_AA_LOCAL;

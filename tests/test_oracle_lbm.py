@@ -83,3 +83,30 @@ def test_poiseuille_pipe(oracle):
     (a, b), *_ = np.linalg.lstsq(A, prof, rcond=None)
     assert np.linalg.norm(A @ [a, b] - prof) < 0.02 * np.linalg.norm(prof)
     assert R - 1.0 < np.sqrt(-a / b) < R + 1.0
+
+
+def shear_wave(oracle, q, dims, tau, amp=1e-3):
+    """Fully periodic box with u_x = amp sin(2 pi y / ny) at equilibrium."""
+    c, w, _ = oracle.lattice(q)
+    nx, ny, nz = dims
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ux = (amp * np.sin(2 * np.pi * j / ny)).reshape(-1)
+    cu = c[:, 0][:, None] * ux[None, :]
+    f = w[:, None] * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * ux * ux)
+    return np.ones(nx * ny * nz, np.uint8), f, ux
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_periodic_shear_wave_decays_at_the_lattice_viscosity(oracle, q):
+    """Closed form: u_x(y, t) = U sin(k y) exp(-nu k^2 t), nu = (tau - 1/2)/3."""
+    dims, tau, steps = (4, 32, 4), 0.8, 200
+    flags, f0, ux0 = shear_wave(oracle, q, dims, tau)
+    s = oracle.Sim(q, dims, flags, tau, [], quirks=1, periodic=7)
+    s.set_f(f0)
+    s.step(steps)
+    _, u = s.macros()
+    kk = 2 * np.pi / dims[1]
+    nu = (tau - 0.5) / 3.0
+    ratio = (u[0] @ ux0) / (ux0 @ ux0)
+    assert abs(ratio - np.exp(-nu * kk * kk * steps)) < 2e-3
+    assert np.abs(u[1:]).max() < 1e-12

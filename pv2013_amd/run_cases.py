@@ -1,0 +1,140 @@
+"""Runs the non-headline benchmark configurations on the GPU and prints one
+JSON line per case (the headline 256x256x512 case is bench.py).
+
+  python -m pv2013_amd.run_cases c3 c4 c5 [--quick]
+
+c3  voxelizer only: seeded tortuous tube (~540k facets) into a 512^3 grid;
+    voxels/s of the voxelizer kernel; flags checked bit-for-bit against the CPU
+    oracle on sampled column blocks (only where the oracle is importable, i.e.
+    in this repository's test environment).
+c4  64 seeded straight and bent pipes (lengths 64-512, radii 8-40, curvature
+    radius 100-1000), FP32 storage, each with its own pulsatile inlet, packed
+    into one domain (one launch per step, hipGraph chunks), 4000 steps:
+    aggregate MLUPS and per-case velocity error against the same cases in FP64.
+c5  D3Q27 768x384x384 vessel tree (3 bifurcation levels, radii 60 -> 15), FP16
+    shifted storage with FP32 collision, 10000 steps: MLUPS; wall shear stress
+    error against an FP64 run, both after 2000 steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import capi as P
+from . import cases
+
+HBM = 8000.0
+
+
+def run_c3(ctx, quick):
+    n = 256 if quick else 512
+    t0 = time.perf_counter()
+    tri, g = cases.c3_mesh(seed=3, n=n)
+    gen = time.perf_counter() - t0
+    flags, ms = ctx.voxelize(tri, g)
+    flags2, ms2 = ctx.voxelize(tri, g)  # second call: warm
+    out = {"case": "c3", "grid": [n, n, n], "facets": len(tri), "generate_s": gen,
+           "kernel_ms": min(ms, ms2), "voxels_per_s": n ** 3 / (min(ms, ms2) * 1e-3),
+           "fluid_fraction": float(flags.mean()), "repeat_identical": bool(np.array_equal(flags, flags2))}
+    try:
+        sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+        import pyoracle as O
+        rng = np.random.default_rng(0)
+        ok = True
+        blocks = []
+        for _ in range(3):
+            i0, j0 = (int(v) for v in rng.integers(0, n - 16, 2))
+            og = O.make_grid((float(i0), float(j0), 0.0), 1.0, (16, 16, n), tuple(g.raybox))
+            sub = O.voxelize(tri, og, 2).reshape(n, 16, 16)
+            mine = flags.reshape(n, n, n)[:, j0:j0 + 16, i0:i0 + 16]
+            ok &= bool(np.array_equal(sub, mine))
+            blocks.append([i0, j0])
+        out["oracle_blocks_bit_exact"] = ok
+        out["oracle_blocks"] = blocks
+    except ImportError:
+        out["oracle_blocks_bit_exact"] = None
+    return out
+
+
+def run_c4(ctx, quick):
+    ncases = 8 if quick else 64
+    steps = 400 if quick else 4000
+    t0 = time.perf_counter()
+    cs = cases.c4(ctx, seed=4, ncases=ncases)
+    gen = time.perf_counter() - t0
+    res = {}
+    for name, st in (("f32", P.STORE_F32), ("f64", P.STORE_F64)):
+        dom, info = P.batch(ctx, [dict(dims=c["dims"], flags=c["flags"], tau=c["tau"],
+                                       bcs=[P.make_bc(**b) for b in c["bcs"]], q=19, storage=st)
+                                  for c in cs])
+        for c, inf in zip(cs, info):
+            for slot, (kind, period, a, phi) in c["waves"].items():
+                dom.set_waveform(inf["wave_base"] + slot, kind, period, a, phi)
+        stats = dom.stats()
+        dom.step(64)
+        ms = dom.step_timed(steps - 64)
+        per_case = [dom.macros_box(inf["origin"], inf["dims"])[1] for inf in info]
+        res[name] = {"mlups": stats["nodes"] * (steps - 64) / (ms * 1e-3) / 1e6, "nodes": stats["nodes"],
+                     "u": per_case, "regular_wave_fraction": stats["regular_waves"] / max(1, stats["waves"])}
+        dom.close()
+    errs = [cases.errors(u64, u32, c["flags"] > 0)["l2"]
+            for u64, u32, c in zip(res["f64"]["u"], res["f32"]["u"], cs)]
+    # the packed FP64 run equals each case run alone, bit for bit: check one
+    c = cs[0]
+    single = cases.make_domain(ctx, c, storage=P.STORE_F64)
+    single.step(steps)
+    same = bool(np.array_equal(single.macros()[1], res["f64"]["u"][0]))
+    single.close()
+    return {"case": "c4", "cases": ncases, "steps": steps, "generate_s": gen, "fluid_nodes": res["f32"]["nodes"],
+            "mlups_f32": res["f32"]["mlups"], "mlups_f64": res["f64"]["mlups"],
+            "roofline_frac_f32": res["f32"]["mlups"] * 152 / 1e3 / HBM,
+            "velocity_l2_vs_f64": {"max": max(errs), "median": float(np.median(errs))},
+            "packed_f64_equals_single_case": same,
+            "regular_wave_fraction": res["f32"]["regular_wave_fraction"]}
+
+
+def run_c5(ctx, quick):
+    dims = (384, 192, 192) if quick else (768, 384, 384)
+    steps = 1000 if quick else 10000
+    cmp_steps = 500 if quick else 2000
+    t0 = time.perf_counter()
+    c = cases.c5(ctx, seed=5, dims=dims, h=4.0 if quick else 3.0)
+    gen = time.perf_counter() - t0
+    dom = cases.make_domain(ctx, c, q=27, storage=P.STORE_F16S)
+    st = dom.stats()
+    dom.step(cmp_steps)
+    w16 = dom.wss()
+    ms = dom.step_timed(steps - cmp_steps)
+    mass, _ = dom.reduce()
+    dom.close()
+    ref = cases.make_domain(ctx, c, q=27, storage=P.STORE_F64)
+    ref.step(cmp_steps)
+    w64 = ref.wss()
+    ms64 = ref.step_timed(200)
+    ref.close()
+    wall = c["flags"] == 2
+    mlups = st["nodes"] * (steps - cmp_steps) / (ms * 1e-3) / 1e6
+    return {"case": "c5", "box": list(dims), "facets": c["ntri"], "generate_s": gen, "fluid_nodes": st["nodes"],
+            "steps": steps, "mlups_f16s": mlups, "roofline_frac_f16s": mlups * 108 / 1e3 / HBM,
+            "mlups_f64": st["nodes"] * 200 / (ms64 * 1e-3) / 1e6,
+            "wss_error_vs_f64_at_%d" % cmp_steps: cases.errors(w64, w16, wall), "mass_after": mass}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cases", nargs="+", choices=["c3", "c4", "c5"])
+    ap.add_argument("--quick", action="store_true", help="reduced sizes for a smoke run")
+    args = ap.parse_args()
+    ctx = P.Context(0)
+    for name in args.cases:
+        print("run_cases: %s" % name, file=sys.stderr, flush=True)
+        fn = {"c3": run_c3, "c4": run_c4, "c5": run_c5}[name]
+        print(json.dumps(fn(ctx, args.quick), default=float), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

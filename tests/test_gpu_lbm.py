@@ -223,7 +223,7 @@ def test_aa_fp64_matches_oracle_six_faces(P, oracle, ctx):
 @pytest.mark.parametrize("scheme", [0, 1])
 def test_periodic_shear_wave(P, oracle, ctx, q, scheme):
     """Periodic axes (ghost-filled site map) against the oracle's wrapped
-    neighbours: FP64 bitwise, FP32 to 1e-6; the wave decays at nu = (tau-1/2)/3."""
+    neighbours: FP64 bitwise, FP32 to 1e-5 of the amplitude; the wave decays at nu = (tau-1/2)/3."""
     from test_oracle_lbm import shear_wave
     dims, tau, steps = (6, 32, 5), 0.8, 150
     flags, f0, ux0 = shear_wave(oracle, q, dims, tau)
@@ -240,7 +240,7 @@ def test_periodic_shear_wave(P, oracle, ctx, q, scheme):
         else:
             _, u = dom.macros()
             _, ur = ref.macros()
-            assert np.abs(u - ur).max() < 1e-6 * np.abs(ur).max() + 1e-12
+            assert np.abs(u - ur).max() < 1e-5 * np.abs(ur).max()
     _, u = ref.macros()
     kk = 2 * np.pi / dims[1]
     assert abs((u[0] @ ux0) / (ux0 @ ux0) - np.exp(-(tau - 0.5) / 3 * kk * kk * steps)) < 2e-3

@@ -146,6 +146,7 @@ typedef struct {
 #define LBM_OPT_PERIODIC_X 4
 #define LBM_OPT_PERIODIC_Y 8
 #define LBM_OPT_PERIODIC_Z 16
+#define LBM_OPT_NO_SPLIT 32        /* one launch per step over all nodes instead of boundary + interior */
 
 /* SubDomain(Geometry&) (Subdomain.cpp:3-33: CreateNodes, Connectivity,
  * Initialize, GetFluidHaloLat) + Simulation's constructor set-up

@@ -18,6 +18,7 @@ SCHEME_TWO_LATTICE, SCHEME_AA = 0, 1
 OPT_NO_ROW_FASTPATH = 1
 OPT_NO_GRAPH = 2
 OPT_PERIODIC_X, OPT_PERIODIC_Y, OPT_PERIODIC_Z = 4, 8, 16
+OPT_NO_SPLIT = 32
 BC_VELOCITY, BC_PRESSURE = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 WAVE_NONE, WAVE_FOURIER, WAVE_PV2013 = 0, 1, 2

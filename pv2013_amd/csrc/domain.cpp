@@ -109,6 +109,7 @@ struct lbm_domain {
     uint8_t* btype = nullptr;
     uint32_t* bbb = nullptr;
     uint32_t* bmiss = nullptr;
+    int32_t* rec_node = nullptr;
     double* bprof = nullptr;
     void* bstale = nullptr;
     FaceTab* d_faces = nullptr;
@@ -237,6 +238,7 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.btype = D->btype;
     a.bbb = D->bbb;
     a.bmiss = D->bmiss;
+    a.rec_node = D->rec_node;
     a.bprof = D->bprof;
     a.bstale = D->bstale;
     a.faces = D->d_faces;
@@ -266,8 +268,9 @@ static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* 
     a.fpre = D->fpre;
     a.rho = D->rho;
     a.u = D->u;
-    hipError_t e = D->storage == LBM_STORE_F64 ? launch_step_f64(mode, sch, a, ctx->stream)
-                                               : launch_step_lp(D->storage, mode, sch, a, ctx->stream);
+    const bool split = !(D->options & LBM_OPT_NO_SPLIT);
+    hipError_t e = D->storage == LBM_STORE_F64 ? launch_step_f64(mode, sch, split, a, ctx->stream)
+                                               : launch_step_lp(D->storage, mode, sch, split, a, ctx->stream);
     CHECK(e);
     return LBM_OK;
 }
@@ -411,6 +414,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     std::vector<int32_t> pre((size_t)nwords, 0);
     std::vector<uint8_t> btype;
     std::vector<uint32_t> bbb, bmiss;
+    std::vector<int32_t> rec_node;
     std::vector<double> bprof;
     std::vector<uint32_t> bstale_mask;
     D->faces.resize(std::max(d.nbc, 1));
@@ -451,6 +455,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
         btype.push_back((uint8_t)type);
         bbb.push_back(bb);
         bmiss.push_back(miss);
+        rec_node.push_back((int32_t)m);
         bstale_mask.push_back(miss & ~bb);
         bprof.push_back(type >= 3 ? profile_factor(d.bcs[type - 3], ijk) : 1.0);
     }
@@ -501,6 +506,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->btype, (size_t)D->nrec));
     DCHECK(D->alloc(&D->bbb, (size_t)D->nrec));
     DCHECK(D->alloc(&D->bmiss, (size_t)D->nrec));
+    DCHECK(D->alloc(&D->rec_node, (size_t)D->nrec));
     DCHECK(D->alloc(&D->bprof, (size_t)D->nrec));
     uint8_t* stale_raw = nullptr;
     DCHECK(D->alloc(&stale_raw, (size_t)q * std::max<int64_t>(D->nrec, 1) * stale_sb));
@@ -531,6 +537,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if ((rc = upload(D, D->btype, btype.data(), btype.size()))) return fail(rc);
     if ((rc = upload(D, D->bbb, bbb.data(), sizeof(uint32_t) * bbb.size()))) return fail(rc);
     if ((rc = upload(D, D->bmiss, bmiss.data(), sizeof(uint32_t) * bmiss.size()))) return fail(rc);
+    if ((rc = upload(D, D->rec_node, rec_node.data(), sizeof(int32_t) * rec_node.size()))) return fail(rc);
     if ((rc = upload(D, D->bprof, bprof.data(), sizeof(double) * bprof.size()))) return fail(rc);
     if ((rc = upload(D, D->d_faces, D->faces.data(), sizeof(FaceTab) * D->faces.size()))) return fail(rc);
     if ((rc = upload(D, D->d_waves, D->waves.data(), sizeof(WaveTab) * D->waves.size()))) return fail(rc);

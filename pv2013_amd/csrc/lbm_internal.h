@@ -45,6 +45,7 @@ struct StepArgs {
     const uint8_t* btype;   // per record: flag value (1 fluid, 2 wall, 3+b)
     const uint32_t* bbb;    // per record: populations taken by bounce-back
     const uint32_t* bmiss;  // per record: populations whose source node is missing
+    const int32_t* rec_node;  // per record: its node id
     const double* bprof;    // per record: velocity profile factor
     const void* bstale;     // [e][r] stale populations (double, or float deviations)
     const FaceTab* faces;
@@ -88,14 +89,15 @@ enum { LBM_MODE_STEP = 0, LBM_MODE_PRE = 1, LBM_MODE_SNAP = 2 };
 hipError_t launch_advance_iter(int64_t* iter, int64_t by, hipStream_t s);
 hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s);
 // sch: 0 two-lattice pull, 1 AA gather step, 2 AA local step
-hipError_t launch_step_f64(int mode, int sch, const StepArgs& a, hipStream_t s);
+// split: boundary records and interior nodes in two launches
+hipError_t launch_step_f64(int mode, int sch, bool split, const StepArgs& a, hipStream_t s);
 hipError_t launch_init_f64(int q, bool swap, const InitArgs& a, hipStream_t s);
 hipError_t launch_wss(int q, const WssArgs& a, hipStream_t s);
 hipError_t launch_reduce(const double* rho, const double* u, int64_t n, const int32_t* bnode,
                          const int32_t* bface, int64_t nbnd, const FaceTab* faces, int nbc,
                          double* out /* 1 + nbc + 1 */, hipStream_t s);
 // kernels_lp.hip (storage LBM_STORE_F32 or LBM_STORE_F16S)
-hipError_t launch_step_lp(int storage, int mode, int sch, const StepArgs& a, hipStream_t s);
+hipError_t launch_step_lp(int storage, int mode, int sch, bool split, const StepArgs& a, hipStream_t s);
 hipError_t launch_init_lp(int storage, int q, bool swap, const InitArgs& a, hipStream_t s);
 
 // Grid rounded up to whole groups of 8 blocks so the XCD-aware block remap is a

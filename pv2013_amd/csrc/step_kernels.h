@@ -382,23 +382,46 @@ __device__ __forceinline__ void write_pre(const StepArgs& a, int64_t n, const ty
 
 // ------------------------------------------------------------------- kernel
 
-template <int Q, class P, int MODE, int SCH>
+// Which nodes a launch updates.
+//   PART_ALL:      every node (one kernel per step)
+//   PART_INTERIOR: nodes without a boundary record; the bounce-back, Zou-He
+//                  and waveform code is compiled out, which keeps registers
+//                  low and wavefronts free of divergent boundary paths
+//   PART_BOUNDARY: one thread per boundary record
+// Interior and boundary launches write disjoint populations, in both
+// streaming schemes, so a step is the two launches in either order.
+enum { PART_ALL = 0, PART_INTERIOR = 1, PART_BOUNDARY = 2 };
+
+template <int Q, class P, int MODE, int SCH, int PART>
 __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     using L = Lat<Q>;
     using S = typename P::store_t;
     using T = typename P::calc_t;
     constexpr bool F64 = sizeof(T) == 8;
-    const int64_t lb = logical_block();
-    const int64_t n = lb * blockDim.x + threadIdx.x;
-    if (lb < a.nblocks && n < a.n) {
-        const S* __restrict__ gin = (const S*)a.gin;
+    int64_t n;
+    int64_t rec = -1;
+    bool active;
+    if constexpr (PART == PART_BOUNDARY) {
+        const int64_t t = logical_block() * blockDim.x + threadIdx.x;
+        active = t < a.nrec;
+        rec = active ? t : 0;
+        n = active ? (int64_t)a.rec_node[t] : 0;
+    } else {
+        const int64_t lb = logical_block();
+        n = lb * blockDim.x + threadIdx.x;
+        active = lb < a.nblocks && n < a.n;
+    }
+    if (active) {
+        // gin and gout are the same buffer in the AA scheme, where a thread
+        // reads and then writes the same slots: no restrict qualifiers.
+        const S* gin = (const S*)a.gin;
         const int64_t N = a.n;
         T f[Q];
         int32_t dst[Q];  // AA gather: destination node of each population (-1: none)
         // Wavefront index, made wave-uniform so the loads below that depend
         // only on it are scalar loads.
         const int wv = __builtin_amdgcn_readfirstlane((int)(n >> 6));
-        bool reg = a.wreg && ((a.wreg[wv >> 5] >> (wv & 31)) & 1u);
+        bool reg = PART != PART_BOUNDARY && a.wreg && ((a.wreg[wv >> 5] >> (wv & 31)) & 1u);
         int32_t b_src[Q], b_dst[Q];
         if (reg) {
             // All 64 lanes are interior nodes of one contiguous row, so for
@@ -433,6 +456,13 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
             }
             if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
         } else {
+            uint64_t word = 0;
+            int bit = 0;
+            if constexpr (PART != PART_BOUNDARY) {
+                word = a.bbits[n >> 6];
+                bit = (int)(n & 63);
+                if (PART == PART_INTERIOR && ((word >> bit) & 1ull)) return;  // a boundary launch owns it
+            }
             if (SCH != SCH_AA_LOCAL) p = a.pnode[n];
 #pragma unroll
             for (int d = 0; d < Q; ++d) {
@@ -446,10 +476,16 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
                 }
             }
             if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
-            const uint64_t word = a.bbits[n >> 6];
-            const int bit = (int)(n & 63);
-            if ((word >> bit) & 1ull) {
-                const int64_t r = a.bpre[n >> 6] + __popcll(word & ((1ull << bit) - 1ull));
+            bool has_rec = false;
+            int64_t r = 0;
+            if constexpr (PART == PART_BOUNDARY) {
+                has_rec = true;
+                r = rec;
+            } else if constexpr (PART == PART_ALL) {
+                has_rec = (word >> bit) & 1ull;
+                if (has_rec) r = a.bpre[n >> 6] + __popcll(word & ((1ull << bit) - 1ull));
+            }
+            if constexpr (PART != PART_INTERIOR) if (has_rec) {
                 const uint32_t bb = a.bbb[r];
                 if (SCH == SCH_AA_LOCAL) miss = a.bmiss[r];
                 if (miss) {
@@ -486,7 +522,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
         }
         if (MODE == LBM_MODE_PRE || MODE == LBM_MODE_SNAP) write_pre<P, Q>(a, n, f);
         if (MODE != LBM_MODE_PRE) {
-            S* __restrict__ gout = (S*)a.gout;
+            S* gout = (S*)a.gout;
             T g[Q];
             if constexpr (F64) {
                 collide_f64<Q>((const double*)f, a.omega, (double*)g);
@@ -548,22 +584,36 @@ inline bool step_args_ok(int mode, int sch, const StepArgs& a) {
     return sch >= SCH_TWO && sch <= SCH_AA_LOCAL;
 }
 
-template <int Q, class P, int MODE>
+template <int Q, class P, int MODE, int PART>
 void launch_mode(int sch, int grid, const StepArgs& b, hipStream_t s) {
-    if (sch == SCH_TWO) hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_TWO>), dim3(grid), dim3(256), 0, s, b);
-    else if (sch == SCH_AA_GATHER) hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_AA_GATHER>), dim3(grid), dim3(256), 0, s, b);
-    else hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_AA_LOCAL>), dim3(grid), dim3(256), 0, s, b);
+    if (sch == SCH_TWO) hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_TWO, PART>), dim3(grid), dim3(256), 0, s, b);
+    else if (sch == SCH_AA_GATHER)
+        hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_AA_GATHER, PART>), dim3(grid), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_AA_LOCAL, PART>), dim3(grid), dim3(256), 0, s, b);
 }
 
+template <int Q, class P, int PART>
+void launch_part(int mode, int sch, int grid, const StepArgs& b, hipStream_t s) {
+    if (mode == LBM_MODE_STEP) launch_mode<Q, P, LBM_MODE_STEP, PART>(sch, grid, b, s);
+    else if (mode == LBM_MODE_SNAP) launch_mode<Q, P, LBM_MODE_SNAP, PART>(sch, grid, b, s);
+    else hipLaunchKernelGGL((k_step<Q, P, LBM_MODE_PRE, SCH_TWO, PART>), dim3(grid), dim3(256), 0, s, b);
+}
+
+// split: one launch over the boundary records, one over the interior nodes
+// (the boundary launch goes first: it is small and its tail overlaps the
+// start of the interior launch); otherwise one launch over all nodes.
 template <int Q, class P>
-hipError_t launch_step_t(int mode, int sch, const StepArgs& a, hipStream_t s) {
+hipError_t launch_step_t(int mode, int sch, bool split, const StepArgs& a, hipStream_t s) {
     if (!step_args_ok(mode, sch, a)) return hipErrorInvalidValue;
-    const int grid = lbm_grid_blocks(a.n, 256);
+    if (split && a.nrec > 0 && !a.rec_node) return hipErrorInvalidValue;
     StepArgs b = a;
     b.nblocks = (int)((a.n + 255) / 256);
-    if (mode == LBM_MODE_STEP) launch_mode<Q, P, LBM_MODE_STEP>(sch, grid, b, s);
-    else if (mode == LBM_MODE_SNAP) launch_mode<Q, P, LBM_MODE_SNAP>(sch, grid, b, s);
-    else hipLaunchKernelGGL((k_step<Q, P, LBM_MODE_PRE, SCH_TWO>), dim3(grid), dim3(256), 0, s, b);
+    if (!split) {
+        launch_part<Q, P, PART_ALL>(mode, sch, lbm_grid_blocks(a.n, 256), b, s);
+        return hipGetLastError();
+    }
+    if (a.nrec > 0) launch_part<Q, P, PART_BOUNDARY>(mode, sch, lbm_grid_blocks(a.nrec, 256), b, s);
+    launch_part<Q, P, PART_INTERIOR>(mode, sch, lbm_grid_blocks(a.n, 256), b, s);
     return hipGetLastError();
 }
 

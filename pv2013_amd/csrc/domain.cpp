@@ -755,9 +755,29 @@ __global__ void k_moments_from_f(const double* fpre, int64_t n, double* rho, dou
     rho[i] = r;
 }
 
+// Host copies of device state are ordered on the context stream (a non-blocking
+// stream: the legacy NULL stream does not wait for it) and complete on return.
+static int copy_d2h(lbm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return LBM_OK;
+    CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    CHECK(hipStreamSynchronize(ctx->stream));
+    return LBM_OK;
+}
+
+static int copy_h2d(lbm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return LBM_OK;
+    CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    CHECK(hipStreamSynchronize(ctx->stream));
+    return LBM_OK;
+}
+
 static int ensure_pre(lbm_domain* D) {
     lbm_ctx* ctx = D->ctx;
-    if (D->pre_iter == D->iter) return LBM_OK;
+    if (D->pre_iter == D->iter) {
+        // the AA scheme writes the snapshot with its last step: wait for it
+        CHECK(hipStreamSynchronize(ctx->stream));
+        return LBM_OK;
+    }
     int rc = ensure_pre_buffers(D);
     if (rc) return rc;
     const unsigned grid = (unsigned)((D->n + 255) / 256);
@@ -781,7 +801,8 @@ static int ensure_pre(lbm_domain* D) {
 static int scatter_dense(lbm_domain* D, const double* dev, int ncomp, double* out) {
     lbm_ctx* ctx = D->ctx;
     std::vector<double> h((size_t)ncomp * D->n);
-    CHECK(hipMemcpy(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    int rc = copy_d2h(ctx, h.data(), dev, sizeof(double) * h.size());
+    if (rc) return rc;
     std::memset(out, 0, sizeof(double) * ncomp * D->nsites);
     for (int c = 0; c < ncomp; ++c)
         for (int64_t m = 0; m < D->n; ++m) out[c * D->nsites + D->node_site[m]] = h[c * D->n + m];
@@ -820,13 +841,13 @@ extern "C" int lbm_set_f(lbm_domain* D, const double* f) {
     std::vector<double> h((size_t)q * D->n);
     for (int e = 0; e < q; ++e)
         for (int64_t m = 0; m < D->n; ++m) h[e * D->n + m] = f[e * D->nsites + D->node_site[m]];
-    CHECK(hipMemcpy(D->fpre, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    if ((rc = copy_h2d(ctx, D->fpre, h.data(), sizeof(double) * h.size()))) return rc;
     // stale populations take the new values (the reference never rewrites them)
     std::vector<double> stale((size_t)q * std::max<int64_t>(D->nrec, 1), 0.0);
     {
         const int64_t nwords = (D->n + 63) / 64;
         std::vector<uint64_t> bits((size_t)nwords);
-        CHECK(hipMemcpy(bits.data(), D->bbits, sizeof(uint64_t) * nwords, hipMemcpyDeviceToHost));
+        if ((rc = copy_d2h(ctx, bits.data(), D->bbits, sizeof(uint64_t) * nwords))) return rc;
         int64_t r = 0;
         for (int64_t m = 0; m < D->n; ++m)
             if ((bits[m >> 6] >> (m & 63)) & 1ull) {
@@ -852,7 +873,7 @@ extern "C" int lbm_get_wss(lbm_domain* D, double* wss) {
     int rc = compute_wss(D);
     if (rc) return rc;
     std::vector<double> h(std::max<size_t>(D->wall_nodes.size(), 1));
-    CHECK(hipMemcpy(h.data(), D->wss, sizeof(double) * D->wall_nodes.size(), hipMemcpyDeviceToHost));
+    if ((rc = copy_d2h(ctx, h.data(), D->wss, sizeof(double) * D->wall_nodes.size()))) return rc;
     std::memset(wss, 0, sizeof(double) * D->nsites);
     for (size_t i = 0; i < D->wall_nodes.size(); ++i) wss[D->node_site[D->wall_nodes[i]]] = h[i];
     return LBM_OK;
@@ -893,8 +914,8 @@ static int scatter_box(lbm_domain* D, const double* dev, int ncomp, const int32_
     };
     if (subset) {
         std::vector<double> h(std::max<size_t>(subset->size(), 1));
-        if (!subset->empty())
-            CHECK(hipMemcpy(h.data(), dev, sizeof(double) * subset->size(), hipMemcpyDeviceToHost));
+        int rc = copy_d2h(ctx, h.data(), dev, sizeof(double) * subset->size());
+        if (rc) return rc;
         for (size_t i = 0; i < subset->size(); ++i) {
             int64_t l;
             if (inside(D->node_site[(*subset)[i]], &l)) out[l] = h[i];
@@ -902,7 +923,8 @@ static int scatter_box(lbm_domain* D, const double* dev, int ncomp, const int32_
         return LBM_OK;
     }
     std::vector<double> h((size_t)ncomp * D->n);
-    CHECK(hipMemcpy(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    int rc = copy_d2h(ctx, h.data(), dev, sizeof(double) * h.size());
+    if (rc) return rc;
     for (int64_t m = 0; m < D->n; ++m) {
         int64_t l;
         if (!inside(D->node_site[m], &l)) continue;

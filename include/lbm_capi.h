@@ -36,6 +36,9 @@ extern "C" {
 #define LBM_ERANGE (-34)  /* size beyond a stated limit */
 #define LBM_ENAN (-61)    /* non-finite state found by a health check */
 
+#define LBM_MAX_BCS 253        /* boundary conditions per domain (flag values 3..255) */
+#define LBM_MAX_WAVEFORMS 256  /* waveform slots per domain */
+
 typedef struct lbm_ctx lbm_ctx;
 typedef struct lbm_domain lbm_domain;
 

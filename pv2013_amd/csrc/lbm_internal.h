@@ -9,8 +9,8 @@
 
 #include "lbm_capi.h"
 
-constexpr int LBM_MAXBC = 253;   // flag values 3 .. 255
-constexpr int LBM_MAXWAVE = 256;
+constexpr int LBM_MAXBC = LBM_MAX_BCS;
+constexpr int LBM_MAXWAVE = LBM_MAX_WAVEFORMS;
 constexpr int LBM_MAXK = 16;
 
 // One Zou-He boundary condition as the kernels see it.  The population index

@@ -30,9 +30,9 @@ SEED = 2013
 PERIOD = 4000.0
 
 
-def run(P, ctx, case, storage, steps, warmup, scheme=0):
+def run(P, ctx, case, storage, steps, warmup, scheme=0, options=0):
     from pv2013_amd.cases import make_domain
-    dom = make_domain(ctx, case, storage=storage, scheme=scheme)
+    dom = make_domain(ctx, case, storage=storage, scheme=scheme, options=options)
     st = dom.stats()
     n = st["nodes"]
     dom.step(warmup)
@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64 / FP16 comparison runs and the accuracy protocol")
     ap.add_argument("--scheme", default="two", choices=["two", "aa"])
+    ap.add_argument("--layout", default="fused", choices=["fused", "split", "generic"],
+                    help="step kernel: records + row segments in one launch (default) or two, "
+                         "or the generic per-node kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the whole domain runs on a single MI355X "
@@ -123,7 +126,8 @@ def main():
     print("bench: timing %d steps (%s) after %d warm-up" % (args.steps, args.storage, args.warmup),
           file=sys.stderr, flush=True)
     scheme = P.SCHEME_AA if args.scheme == "aa" else P.SCHEME_TWO_LATTICE
-    r = run(P, ctx, case, storage, args.steps, args.warmup, scheme=scheme)
+    options = {"fused": 0, "split": P.capi.OPT_SPLIT_LAUNCH, "generic": P.capi.OPT_GENERIC_KERNEL}[args.layout]
+    r = run(P, ctx, case, storage, args.steps, args.warmup, scheme=scheme, options=options)
     n = r["nodes"]
     ms_step = r["ms"] / args.steps
     mlups = n * args.steps / (r["ms"] * 1e-3) / 1e6
@@ -133,14 +137,18 @@ def main():
              "wall_s_timed": r["wall_s"], "mass_after": r["mass"], "facets": case["ntri"],
              "voxelize_ms": case["vox_ms"], "classify_ms": case["cls_ms"], "scheme": args.scheme,
              "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3),
-             "regular_wave_fraction": r["stats"]["regular_waves"] / max(1, r["stats"]["waves"]),
-             "boundary_records": r["stats"]["boundary_records"], "node_order_fast_axis": r["stats"]["fast_axis"]}
+             "layout": args.layout, "boundary_records": r["stats"]["boundary_records"],
+             "segments": r["stats"]["segments"],
+             "segment_lane_use": (n - r["stats"]["boundary_records"]) / max(1, 64 * r["stats"]["segments"]),
+             "node_order_fast_axis": r["stats"]["fast_axis"]}
+    if args.layout == "generic":
+        extra["regular_wave_fraction"] = r["stats"]["regular_waves"] / max(1, r["stats"]["waves"])
     if not args.no_compare:
         for name, st in (("f64", P.STORE_F64), ("f16s", P.STORE_F16S)):
             if name == args.storage:
                 continue
             print("bench: comparison run %s" % name, file=sys.stderr, flush=True)
-            rr = run(P, ctx, case, st, min(args.steps, 1000), min(args.warmup, 100))
+            rr = run(P, ctx, case, st, min(args.steps, 1000), min(args.warmup, 100), scheme=scheme, options=options)
             extra["mlups_" + name] = rr["nodes"] * min(args.steps, 1000) / (rr["ms"] * 1e-3) / 1e6
         if "mlups_f64" in extra:
             extra["speedup_vs_f64"] = mlups / extra["mlups_f64"]

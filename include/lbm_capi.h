@@ -140,7 +140,15 @@ typedef struct {
 } lbm_domain_desc;
 
 /* lbm_domain_desc.options */
-#define LBM_OPT_NO_ROW_FASTPATH 1  /* address every lane through the site map */
+/* Step kernels.  Default: one launch per step in which boundary records (nodes
+ * with a missing link or a boundary condition) take one thread each and every
+ * other node belongs to a row segment (up to 64 consecutive node ids whose
+ * sources per direction are consecutive ids) read with contiguous loads.
+ * LBM_OPT_GENERIC_KERNEL: one thread per node, sources through the site map,
+ * with a scalar fast path for whole interior wavefronts (off with
+ * LBM_OPT_NO_ROW_FASTPATH, which implies the generic kernel).  Both give
+ * identical results; they share no addressing code. */
+#define LBM_OPT_NO_ROW_FASTPATH 1  /* generic kernel, every lane through the site map */
 #define LBM_OPT_NO_GRAPH 2         /* launch every step eagerly instead of replaying a hipGraph */
 /* Periodic axes: the halo of the site map holds the wrapped node ids (a ghost
  * fill done once at build time), so streaming wraps at no cost per step.  The
@@ -149,7 +157,8 @@ typedef struct {
 #define LBM_OPT_PERIODIC_X 4
 #define LBM_OPT_PERIODIC_Y 8
 #define LBM_OPT_PERIODIC_Z 16
-#define LBM_OPT_NO_SPLIT 32        /* one launch per step over all nodes instead of boundary + interior */
+#define LBM_OPT_GENERIC_KERNEL 32  /* generic per-node kernel instead of records + segments */
+#define LBM_OPT_SPLIT_LAUNCH 64    /* records and segments in two launches per step */
 
 /* SubDomain(Geometry&) (Subdomain.cpp:3-33: CreateNodes, Connectivity,
  * Initialize, GetFluidHaloLat) + Simulation's constructor set-up
@@ -162,10 +171,13 @@ int lbm_domain_info(const lbm_domain* dom, int64_t* n_nodes, int64_t* n_boundary
 
 typedef struct {
     int64_t nodes, boundary_records, wall_nodes, bc_nodes;
-    int64_t waves, regular_waves;  /* 64-node wavefronts; those on the row fast path */
+    int64_t waves, regular_waves;  /* generic kernel: 64-node wavefronts; those on the row fast path */
+    int64_t segments;              /* record/segment kernel: row segments (0 for the generic kernel) */
     int64_t device_bytes;
     int32_t fast_axis;             /* node-order fastest axis */
     int32_t scheme;
+    int32_t layout;                /* 0 generic kernel, 1 records + segments in one launch, 2 in two */
+    int32_t pad_;
 } lbm_domain_stats;
 int lbm_domain_get_stats(const lbm_domain* dom, lbm_domain_stats* out);
 
@@ -214,7 +226,8 @@ typedef struct {
 } lbm_tune_result;
 
 /* Times each candidate layout of `desc` (streaming scheme two-lattice / AA,
- * node-order fast axis x / y / z, row fast path on / off) for `steps` steps
+ * node-order fast axis x / y / z, kernel: records + segments in one launch /
+ * in two / generic) for `steps` steps
  * after a warm-up and returns the fastest.  When cache_path is not NULL the
  * result is looked up there first (key: lattice, storage, dims, flag hash) and
  * appended after tuning.  No reference counterpart. */

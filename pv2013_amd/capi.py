@@ -18,7 +18,8 @@ SCHEME_TWO_LATTICE, SCHEME_AA = 0, 1
 OPT_NO_ROW_FASTPATH = 1
 OPT_NO_GRAPH = 2
 OPT_PERIODIC_X, OPT_PERIODIC_Y, OPT_PERIODIC_Z = 4, 8, 16
-OPT_NO_SPLIT = 32
+OPT_GENERIC_KERNEL = 32  # one thread per node through the site map
+OPT_SPLIT_LAUNCH = 64  # records and row segments in two launches
 BC_VELOCITY, BC_PRESSURE = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 WAVE_NONE, WAVE_FOURIER, WAVE_PV2013 = 0, 1, 2
@@ -68,7 +69,8 @@ class DomainDesc(C.Structure):
 class DomainStats(C.Structure):
     _fields_ = [("nodes", C.c_int64), ("boundary_records", C.c_int64), ("wall_nodes", C.c_int64),
                 ("bc_nodes", C.c_int64), ("waves", C.c_int64), ("regular_waves", C.c_int64),
-                ("device_bytes", C.c_int64), ("fast_axis", C.c_int32), ("scheme", C.c_int32)]
+                ("segments", C.c_int64), ("device_bytes", C.c_int64), ("fast_axis", C.c_int32),
+                ("scheme", C.c_int32), ("layout", C.c_int32), ("pad_", C.c_int32)]
 
 
 class TuneResult(C.Structure):

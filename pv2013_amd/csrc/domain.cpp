@@ -106,6 +106,10 @@ struct lbm_domain {
     uint64_t* bbits = nullptr;
     uint32_t* wreg = nullptr;  // null when the row fast path is off
     int64_t nreg_waves = 0;
+    int layout = LBM_LAYOUT_FUSED;
+    int32_t* rsrc = nullptr;   // record/segment layouts: [d][r] record sources
+    int32_t* seg = nullptr;    // [s][LBM_SEG_W] row segments
+    int64_t nseg = 0;
     uint8_t* btype = nullptr;
     uint32_t* bbb = nullptr;
     uint32_t* bmiss = nullptr;
@@ -239,6 +243,9 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.bbb = D->bbb;
     a.bmiss = D->bmiss;
     a.rec_node = D->rec_node;
+    a.rsrc = D->rsrc;
+    a.seg = D->seg;
+    a.nseg = D->nseg;
     a.bprof = D->bprof;
     a.bstale = D->bstale;
     a.faces = D->d_faces;
@@ -268,9 +275,8 @@ static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* 
     a.fpre = D->fpre;
     a.rho = D->rho;
     a.u = D->u;
-    const bool split = !(D->options & LBM_OPT_NO_SPLIT);
-    hipError_t e = D->storage == LBM_STORE_F64 ? launch_step_f64(mode, sch, split, a, ctx->stream)
-                                               : launch_step_lp(D->storage, mode, sch, split, a, ctx->stream);
+    hipError_t e = D->storage == LBM_STORE_F64 ? launch_step_f64(mode, sch, D->layout, a, ctx->stream)
+                                               : launch_step_lp(D->storage, mode, sch, D->layout, a, ctx->stream);
     CHECK(e);
     return LBM_OK;
 }
@@ -468,7 +474,10 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     // Wavefronts (64 consecutive node ids) made of interior nodes only, all in
     // one row at consecutive positions along the fast axis: then for every
     // direction the 64 sources are fluid, hence one contiguous run of ids.
-    const bool fastpath = !(d.options & LBM_OPT_NO_ROW_FASTPATH);
+    D->layout = (d.options & (LBM_OPT_GENERIC_KERNEL | LBM_OPT_NO_ROW_FASTPATH))
+                    ? LBM_LAYOUT_GENERIC
+                    : ((d.options & LBM_OPT_SPLIT_LAUNCH) ? LBM_LAYOUT_SPLIT : LBM_LAYOUT_FUSED);
+    const bool fastpath = D->layout == LBM_LAYOUT_GENERIC && !(d.options & LBM_OPT_NO_ROW_FASTPATH);
     std::vector<uint32_t> wreg((size_t)((nwords + 31) / 32), 0u);
     if (fastpath) {
         for (int64_t w = 0; w < nwords; ++w) {
@@ -480,6 +489,51 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             wreg[w >> 5] |= 1u << (w & 31);
             D->nreg_waves++;
         }
+    }
+    // Record sources and row segments (record/segment kernel).  A segment
+    // grows node by node while the next node is interior, follows the last
+    // one along the fast axis, and its source in every direction follows the
+    // last one's source; segments also end at multiples of 64 node ids so a
+    // wavefront's stores cover whole cache lines.  Every run this accepts is
+    // contiguous by construction, and the bounds are checked again below.
+    std::vector<int32_t> rsrc, segs;
+    if (D->layout != LBM_LAYOUT_GENERIC) {
+        rsrc.assign((size_t)q * std::max<int64_t>(D->nrec, 1), -1);
+        for (int64_t r = 0; r < D->nrec; ++r) {
+            const int32_t m = rec_node[r];
+            for (int e = 0; e < q; ++e) rsrc[e * D->nrec + r] = e == rest ? m : map[pnode[m] - D->off[e]];
+        }
+        int64_t cur = -1;  // index of the open segment
+        for (int64_t m = 0; m < D->n; ++m) {
+            if ((bits[m >> 6] >> (m & 63)) & 1ull) {
+                cur = -1;
+                continue;
+            }
+            bool extend = cur >= 0 && (m & 63) != 0 && segs[cur * LBM_SEG_W + 1] < 64 && pnode[m] == pnode[m - 1] + 1;
+            for (int e = 0; extend && e < q; ++e)
+                if (e != rest && map[pnode[m] - D->off[e]] != map[pnode[m - 1] - D->off[e]] + 1) extend = false;
+            if (extend) {
+                segs[cur * LBM_SEG_W + 1]++;
+                continue;
+            }
+            cur = (int64_t)(segs.size() / LBM_SEG_W);
+            segs.resize(segs.size() + LBM_SEG_W, 0);
+            int32_t* row = &segs[cur * LBM_SEG_W];
+            row[0] = (int32_t)m;
+            row[1] = 1;
+            for (int e = 0; e < q; ++e) row[2 + e] = e == rest ? (int32_t)m : map[pnode[m] - D->off[e]];
+        }
+        D->nseg = (int64_t)(segs.size() / LBM_SEG_W);
+        for (int64_t sg = 0; sg < D->nseg; ++sg) {
+            const int32_t* row = &segs[sg * LBM_SEG_W];
+            bool ok = row[0] >= 0 && row[1] >= 1 && row[1] <= 64 && (int64_t)row[0] + row[1] <= D->n;
+            for (int e = 0; ok && e < q; ++e) ok = row[2 + e] >= 0 && (int64_t)row[2 + e] + row[1] <= D->n;
+            if (!ok) {
+                delete D;
+                return lbm_ctx_fail(ctx, LBM_EIO, "internal error: row segment out of range");
+            }
+        }
+        if (D->nseg == 0) segs.assign(LBM_SEG_W, 0);
     }
     // device allocation
     int rc = LBM_OK;
@@ -508,6 +562,10 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->bmiss, (size_t)D->nrec));
     DCHECK(D->alloc(&D->rec_node, (size_t)D->nrec));
     DCHECK(D->alloc(&D->bprof, (size_t)D->nrec));
+    if (D->layout != LBM_LAYOUT_GENERIC) {
+        DCHECK(D->alloc(&D->rsrc, rsrc.size()));
+        DCHECK(D->alloc(&D->seg, segs.size()));
+    }
     uint8_t* stale_raw = nullptr;
     DCHECK(D->alloc(&stale_raw, (size_t)q * std::max<int64_t>(D->nrec, 1) * stale_sb));
     D->bstale = stale_raw;
@@ -539,6 +597,10 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if ((rc = upload(D, D->bmiss, bmiss.data(), sizeof(uint32_t) * bmiss.size()))) return fail(rc);
     if ((rc = upload(D, D->rec_node, rec_node.data(), sizeof(int32_t) * rec_node.size()))) return fail(rc);
     if ((rc = upload(D, D->bprof, bprof.data(), sizeof(double) * bprof.size()))) return fail(rc);
+    if (D->layout != LBM_LAYOUT_GENERIC) {
+        if ((rc = upload(D, D->rsrc, rsrc.data(), sizeof(int32_t) * rsrc.size()))) return fail(rc);
+        if ((rc = upload(D, D->seg, segs.data(), sizeof(int32_t) * segs.size()))) return fail(rc);
+    }
     if ((rc = upload(D, D->d_faces, D->faces.data(), sizeof(FaceTab) * D->faces.size()))) return fail(rc);
     if ((rc = upload(D, D->d_waves, D->waves.data(), sizeof(WaveTab) * D->waves.size()))) return fail(rc);
     if ((rc = upload(D, D->d_wall, D->wall_nodes.data(), sizeof(int32_t) * D->wall_nodes.size()))) return fail(rc);
@@ -586,6 +648,8 @@ extern "C" int lbm_domain_get_stats(const lbm_domain* D, lbm_domain_stats* o) {
     o->bc_nodes = (int64_t)D->bc_nodes.size();
     o->waves = (D->n + 63) / 64;
     o->regular_waves = D->nreg_waves;
+    o->segments = D->nseg;
+    o->layout = D->layout;
     o->device_bytes = D->device_bytes;
     o->fast_axis = D->perm[0];
     o->scheme = D->scheme;
@@ -1107,11 +1171,14 @@ extern "C" int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps
     double best = -1.0;
     for (int sch = 0; sch < 2; ++sch)
         for (int ax = 0; ax < 3; ++ax)
-            for (int fp = 0; fp < 2; ++fp) {
+            for (int kern = 0; kern < 3; ++kern) {
+                // records + segments in one launch / in two / generic kernel
+                static const int kern_opt[3] = {0, LBM_OPT_SPLIT_LAUNCH, LBM_OPT_GENERIC_KERNEL};
                 lbm_domain_desc d = *desc;
                 d.scheme = sch;
                 d.fast_axis = ax;
-                d.options = (desc->options & ~LBM_OPT_NO_ROW_FASTPATH) | (fp ? LBM_OPT_NO_ROW_FASTPATH : 0);
+                d.options = (desc->options & ~(LBM_OPT_NO_ROW_FASTPATH | LBM_OPT_GENERIC_KERNEL | LBM_OPT_SPLIT_LAUNCH)) |
+                            kern_opt[kern];
                 lbm_domain* D = nullptr;
                 int rc = lbm_domain_create(ctx, &d, &D);
                 if (rc) return rc;

@@ -23,9 +23,9 @@ hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_step_f64(int mode, int sch, bool split, const StepArgs& a, hipStream_t s) {
-    if (a.q == 19) return launch_step_t<19, PrecF64>(mode, sch, split, a, s);
-    return launch_step_t<27, PrecF64>(mode, sch, split, a, s);
+hipError_t launch_step_f64(int mode, int sch, int layout, const StepArgs& a, hipStream_t s) {
+    if (a.q == 19) return launch_step_t<19, PrecF64>(mode, sch, layout, a, s);
+    return launch_step_t<27, PrecF64>(mode, sch, layout, a, s);
 }
 
 hipError_t launch_init_f64(int q, bool swap, const InitArgs& a, hipStream_t s) {

@@ -4,13 +4,13 @@
 
 using namespace lbm;
 
-hipError_t launch_step_lp(int storage, int mode, int sch, bool split, const StepArgs& a, hipStream_t s) {
+hipError_t launch_step_lp(int storage, int mode, int sch, int layout, const StepArgs& a, hipStream_t s) {
     if (storage == LBM_STORE_F32) {
-        if (a.q == 19) return launch_step_t<19, PrecF32S>(mode, sch, split, a, s);
-        return launch_step_t<27, PrecF32S>(mode, sch, split, a, s);
+        if (a.q == 19) return launch_step_t<19, PrecF32S>(mode, sch, layout, a, s);
+        return launch_step_t<27, PrecF32S>(mode, sch, layout, a, s);
     }
-    if (a.q == 19) return launch_step_t<19, PrecF16S>(mode, sch, split, a, s);
-    return launch_step_t<27, PrecF16S>(mode, sch, split, a, s);
+    if (a.q == 19) return launch_step_t<19, PrecF16S>(mode, sch, layout, a, s);
+    return launch_step_t<27, PrecF16S>(mode, sch, layout, a, s);
 }
 
 hipError_t launch_init_lp(int storage, int q, bool swap, const InitArgs& a, hipStream_t s) {

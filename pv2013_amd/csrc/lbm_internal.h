@@ -46,6 +46,10 @@ struct StepArgs {
     const uint32_t* bbb;    // per record: populations taken by bounce-back
     const uint32_t* bmiss;  // per record: populations whose source node is missing
     const int32_t* rec_node;  // per record: its node id
+    const int32_t* rsrc;    // [d][r] source node of record r in direction d (x - c_d), -1: none
+    const int32_t* seg;     // [s][LBM_SEG_W] row segments: start, length, source base per direction
+    int64_t nseg;
+    int32_t rb8, sb8;       // record / segment blocks per XCD (set by the launcher)
     const double* bprof;    // per record: velocity profile factor
     const void* bstale;     // [e][r] stale populations (double, or float deviations)
     const FaceTab* faces;
@@ -85,19 +89,28 @@ struct WssArgs {
 
 enum { LBM_MODE_STEP = 0, LBM_MODE_PRE = 1, LBM_MODE_SNAP = 2 };
 
+// How a step's nodes map onto launches: the generic per-node kernel, or the
+// record/segment kernel in one launch or in two.
+enum { LBM_LAYOUT_GENERIC = 0, LBM_LAYOUT_FUSED = 1, LBM_LAYOUT_SPLIT = 2 };
+
+// Row segment table entry: {start node, length (1..64), base[Q]} padded to 32
+// ints; base[d] is the node id at x_start - c_d, so lane l of the segment reads
+// direction d from node base[d] + l.
+constexpr int LBM_SEG_W = 32;
+
 // kernels_f64.hip
 hipError_t launch_advance_iter(int64_t* iter, int64_t by, hipStream_t s);
 hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s);
 // sch: 0 two-lattice pull, 1 AA gather step, 2 AA local step
-// split: boundary records and interior nodes in two launches
-hipError_t launch_step_f64(int mode, int sch, bool split, const StepArgs& a, hipStream_t s);
+// layout: LBM_LAYOUT_*
+hipError_t launch_step_f64(int mode, int sch, int layout, const StepArgs& a, hipStream_t s);
 hipError_t launch_init_f64(int q, bool swap, const InitArgs& a, hipStream_t s);
 hipError_t launch_wss(int q, const WssArgs& a, hipStream_t s);
 hipError_t launch_reduce(const double* rho, const double* u, int64_t n, const int32_t* bnode,
                          const int32_t* bface, int64_t nbnd, const FaceTab* faces, int nbc,
                          double* out /* 1 + nbc + 1 */, hipStream_t s);
 // kernels_lp.hip (storage LBM_STORE_F32 or LBM_STORE_F16S)
-hipError_t launch_step_lp(int storage, int mode, int sch, bool split, const StepArgs& a, hipStream_t s);
+hipError_t launch_step_lp(int storage, int mode, int sch, int layout, const StepArgs& a, hipStream_t s);
 hipError_t launch_init_lp(int storage, int q, bool swap, const InitArgs& a, hipStream_t s);
 
 // Grid rounded up to whole groups of 8 blocks so the XCD-aware block remap is a

@@ -382,167 +382,235 @@ __device__ __forceinline__ void write_pre(const StepArgs& a, int64_t n, const ty
 
 // ------------------------------------------------------------------- kernel
 
-// Which nodes a launch updates.
-//   PART_ALL:      every node (one kernel per step)
-//   PART_INTERIOR: nodes without a boundary record; the bounce-back, Zou-He
-//                  and waveform code is compiled out, which keeps registers
-//                  low and wavefronts free of divergent boundary paths
-//   PART_BOUNDARY: one thread per boundary record
-// Interior and boundary launches write disjoint populations, in both
-// streaming schemes, so a step is the two launches in either order.
-enum { PART_ALL = 0, PART_INTERIOR = 1, PART_BOUNDARY = 2 };
+// Bounce-back, stale populations and Zou-He of boundary record r on the
+// gathered populations f (miss: links whose source node does not exist).
+template <int Q, class P, int SCH>
+__device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64_t n, uint32_t miss,
+                                             typename P::calc_t* f) {
+    using L = Lat<Q>;
+    using T = typename P::calc_t;
+    const typename P::store_t* gin = (const typename P::store_t*)a.gin;
+    const int64_t N = a.n;
+    const uint32_t bb = a.bbb[r];
+    if (SCH == SCH_AA_LOCAL) miss = a.bmiss[r];
+    if (miss) {
+        const T* stale = (const T*)a.bstale;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            if (!((miss >> d) & 1u)) continue;
+            if ((bb >> d) & 1u) {
+                // bounce-back: this node's own post-collision g[opp d]
+                if (SCH == SCH_TWO) f[d] = load_pop<P>(gin, (int64_t)L::opp(d) * N + n);
+                if (SCH == SCH_AA_GATHER) f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+                // SCH_AA_LOCAL: A[d][x] already holds it (kept by the gather step)
+            } else {
+                f[d] = stale[(int64_t)d * a.nrec + r];
+            }
+        }
+    }
+    const int type = a.btype[r];
+    if (type >= 3) {
+        const FaceTab& F = a.faces[type - 3];
+        double uv[3] = {0.0, 0.0, 0.0};
+        if (F.kind == LBM_BC_VELOCITY) {
+            double w = 1.0;
+            if (F.waveform >= 0) {
+                const int64_t it = a.it_abs ? a.it_off : *a.iter + a.it_off;
+                w = wave_value(a.waves[F.waveform], it);
+            }
+            const double fac = a.bprof[r] * w;
+            uv[0] = F.u[0] * fac; uv[1] = F.u[1] * fac; uv[2] = F.u[2] * fac;
+        }
+        ZouHe<Q, T>::apply(F, uv, f);
+    }
+}
 
-template <int Q, class P, int MODE, int SCH, int PART>
+// Pre-state output (modes PRE / SNAP), collision and store of node n.
+// dst (AA gather only): destination node of each population, -1 if none.
+template <int Q, class P, int MODE, int SCH>
+__device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::calc_t* f, const int32_t* dst) {
+    using L = Lat<Q>;
+    using S = typename P::store_t;
+    using T = typename P::calc_t;
+    if (MODE == LBM_MODE_PRE || MODE == LBM_MODE_SNAP) write_pre<P, Q>(a, n, f);
+    if (MODE == LBM_MODE_PRE) return;
+    const int64_t N = a.n;
+    S* gout = (S*)a.gout;
+    T g[Q];
+    if constexpr (sizeof(T) == 8) {
+        collide_f64<Q>((const double*)f, a.omega, (double*)g);
+    } else {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) g[d] = f[d];
+        collide_dev<Q>((float*)g, (float)a.omega);
+    }
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        if (SCH == SCH_TWO) {
+            gout[(int64_t)d * N + n] = (S)g[d];
+        } else if (SCH == SCH_AA_LOCAL) {
+            gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
+        } else {
+            if (dst[d] >= 0) gout[(int64_t)d * N + dst[d]] = (S)g[d];
+            else gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
+        }
+    }
+}
+
+// Generic kernel: one thread per node, sources found through the padded site
+// map, records through the boundary bitmap.  Wavefronts marked in `wreg` (64
+// interior nodes of one contiguous row) take scalar per-direction bases.  This
+// path shares no addressing with the record/segment kernel below, and the
+// parity tests run both.
+template <int Q, class P, int MODE, int SCH>
 __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     using L = Lat<Q>;
     using S = typename P::store_t;
     using T = typename P::calc_t;
-    constexpr bool F64 = sizeof(T) == 8;
-    int64_t n;
-    int64_t rec = -1;
-    bool active;
-    if constexpr (PART == PART_BOUNDARY) {
-        const int64_t t = logical_block() * blockDim.x + threadIdx.x;
-        active = t < a.nrec;
-        rec = active ? t : 0;
-        n = active ? (int64_t)a.rec_node[t] : 0;
-    } else {
-        const int64_t lb = logical_block();
-        n = lb * blockDim.x + threadIdx.x;
-        active = lb < a.nblocks && n < a.n;
+    const int64_t lb = logical_block();
+    const int64_t n = lb * blockDim.x + threadIdx.x;
+    if (lb >= a.nblocks || n >= a.n) return;
+    // gin and gout are the same buffer in the AA scheme, where a thread
+    // reads and then writes the same slots: no restrict qualifiers.
+    const S* gin = (const S*)a.gin;
+    const int64_t N = a.n;
+    T f[Q];
+    int32_t dst[Q];
+    // Wavefront index, made wave-uniform so the loads below that depend only
+    // on it are scalar loads.
+    const int wv = __builtin_amdgcn_readfirstlane((int)(n >> 6));
+    bool reg = a.wreg && ((a.wreg[wv >> 5] >> (wv & 31)) & 1u);
+    int32_t b_src[Q], b_dst[Q];
+    if (reg) {
+        // All 64 lanes are interior nodes of one contiguous row, so for every
+        // direction their sources (and destinations) are one contiguous run
+        // of node ids.  The wave-uniform bases are checked to lie in
+        // [0, n - 64] before any lane uses them.
+        const int32_t p0 = a.pnode[(int64_t)wv << 6];
+        bool ok = ((int64_t)wv << 6) + 63 < N;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            b_src[d] = (d == L::rest || SCH == SCH_AA_LOCAL) ? (wv << 6) : a.map[p0 - a.off[d]];
+            b_dst[d] = d == L::rest ? (wv << 6) : (SCH == SCH_AA_GATHER ? a.map[p0 + a.off[d]] : 0);
+            ok = ok && b_src[d] >= 0 && (int64_t)b_src[d] + 63 < N && b_dst[d] >= 0 && (int64_t)b_dst[d] + 63 < N;
+        }
+        reg = ok;
     }
-    if (active) {
-        // gin and gout are the same buffer in the AA scheme, where a thread
-        // reads and then writes the same slots: no restrict qualifiers.
-        const S* gin = (const S*)a.gin;
-        const int64_t N = a.n;
-        T f[Q];
-        int32_t dst[Q];  // AA gather: destination node of each population (-1: none)
-        // Wavefront index, made wave-uniform so the loads below that depend
-        // only on it are scalar loads.
-        const int wv = __builtin_amdgcn_readfirstlane((int)(n >> 6));
-        bool reg = PART != PART_BOUNDARY && a.wreg && ((a.wreg[wv >> 5] >> (wv & 31)) & 1u);
-        int32_t b_src[Q], b_dst[Q];
-        if (reg) {
-            // All 64 lanes are interior nodes of one contiguous row, so for
-            // every direction their sources (and destinations) are one
-            // contiguous run of node ids.  The wave-uniform bases are checked
-            // to lie in [0, n - 64] before any lane uses them.
-            const int32_t p0 = a.pnode[(int64_t)wv << 6];
-            bool ok = ((int64_t)wv << 6) + 63 < N;
+    if (reg) {
+        const int lane = (int)(n & 63);
 #pragma unroll
-            for (int d = 0; d < Q; ++d) {
-                b_src[d] = (d == L::rest || SCH == SCH_AA_LOCAL) ? (wv << 6) : a.map[p0 - a.off[d]];
-                b_dst[d] = d == L::rest ? (wv << 6) : (SCH == SCH_AA_GATHER ? a.map[p0 + a.off[d]] : 0);
-                ok = ok && b_src[d] >= 0 && (int64_t)b_src[d] + 63 < N && b_dst[d] >= 0 &&
-                     (int64_t)b_dst[d] + 63 < N;
-            }
-            reg = ok;
-        }
-        int32_t p = 0;
-        uint32_t miss = 0;
-        if (reg) {
-            const int lane = (int)(n & 63);
-#pragma unroll
-            for (int d = 0; d < Q; ++d) {
-                if (SCH == SCH_AA_LOCAL || d == L::rest) {
-                    f[d] = load_pop<P>(gin, (int64_t)d * N + n);
-                    dst[d] = (int32_t)n;
-                } else {
-                    const int64_t src = (int64_t)b_src[d] + lane;
-                    f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
-                    if (SCH == SCH_AA_GATHER) dst[d] = b_dst[d] + lane;
-                }
-            }
-            if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
-        } else {
-            uint64_t word = 0;
-            int bit = 0;
-            if constexpr (PART != PART_BOUNDARY) {
-                word = a.bbits[n >> 6];
-                bit = (int)(n & 63);
-                if (PART == PART_INTERIOR && ((word >> bit) & 1ull)) return;  // a boundary launch owns it
-            }
-            if (SCH != SCH_AA_LOCAL) p = a.pnode[n];
-#pragma unroll
-            for (int d = 0; d < Q; ++d) {
-                if (SCH == SCH_AA_LOCAL || d == L::rest) {
-                    f[d] = load_pop<P>(gin, (int64_t)d * N + n);
-                } else {
-                    const int32_t src = a.map[p - a.off[d]];
-                    if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
-                    else { f[d] = T(0); miss |= 1u << d; }
-                    if (SCH == SCH_AA_GATHER) dst[d] = a.map[p + a.off[d]];
-                }
-            }
-            if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
-            bool has_rec = false;
-            int64_t r = 0;
-            if constexpr (PART == PART_BOUNDARY) {
-                has_rec = true;
-                r = rec;
-            } else if constexpr (PART == PART_ALL) {
-                has_rec = (word >> bit) & 1ull;
-                if (has_rec) r = a.bpre[n >> 6] + __popcll(word & ((1ull << bit) - 1ull));
-            }
-            if constexpr (PART != PART_INTERIOR) if (has_rec) {
-                const uint32_t bb = a.bbb[r];
-                if (SCH == SCH_AA_LOCAL) miss = a.bmiss[r];
-                if (miss) {
-                    const T* stale = (const T*)a.bstale;
-#pragma unroll
-                    for (int d = 0; d < Q; ++d) {
-                        if (!((miss >> d) & 1u)) continue;
-                        if ((bb >> d) & 1u) {
-                            // bounce-back: this node's own post-collision g[opp d]
-                            if (SCH == SCH_TWO) f[d] = load_pop<P>(gin, (int64_t)L::opp(d) * N + n);
-                            if (SCH == SCH_AA_GATHER) f[d] = load_pop<P>(gin, (int64_t)d * N + n);
-                            // SCH_AA_LOCAL: A[d][x] already holds it (kept by the gather step)
-                        } else {
-                            f[d] = stale[(int64_t)d * a.nrec + r];
-                        }
-                    }
-                }
-                const int type = a.btype[r];
-                if (type >= 3) {
-                    const FaceTab& F = a.faces[type - 3];
-                    double uv[3] = {0.0, 0.0, 0.0};
-                    if (F.kind == LBM_BC_VELOCITY) {
-                        double w = 1.0;
-                        if (F.waveform >= 0) {
-                            const int64_t it = a.it_abs ? a.it_off : *a.iter + a.it_off;
-                            w = wave_value(a.waves[F.waveform], it);
-                        }
-                        const double fac = a.bprof[r] * w;
-                        uv[0] = F.u[0] * fac; uv[1] = F.u[1] * fac; uv[2] = F.u[2] * fac;
-                    }
-                    ZouHe<Q, T>::apply(F, uv, f);
-                }
-            }
-        }
-        if (MODE == LBM_MODE_PRE || MODE == LBM_MODE_SNAP) write_pre<P, Q>(a, n, f);
-        if (MODE != LBM_MODE_PRE) {
-            S* gout = (S*)a.gout;
-            T g[Q];
-            if constexpr (F64) {
-                collide_f64<Q>((const double*)f, a.omega, (double*)g);
+        for (int d = 0; d < Q; ++d) {
+            if (SCH == SCH_AA_LOCAL || d == L::rest) {
+                f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+                dst[d] = (int32_t)n;
             } else {
-#pragma unroll
-                for (int d = 0; d < Q; ++d) g[d] = f[d];
-                collide_dev<Q>((float*)g, (float)a.omega);
-            }
-#pragma unroll
-            for (int d = 0; d < Q; ++d) {
-                if (SCH == SCH_TWO) {
-                    gout[(int64_t)d * N + n] = (S)g[d];
-                } else if (SCH == SCH_AA_LOCAL) {
-                    gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
-                } else {
-                    if (dst[d] >= 0) gout[(int64_t)d * N + dst[d]] = (S)g[d];
-                    else gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
-                }
+                const int64_t src = (int64_t)b_src[d] + lane;
+                f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
+                if (SCH == SCH_AA_GATHER) dst[d] = b_dst[d] + lane;
             }
         }
+        if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
+    } else {
+        const uint64_t word = a.bbits[n >> 6];
+        const int bit = (int)(n & 63);
+        uint32_t miss = 0;
+        const int32_t p = SCH != SCH_AA_LOCAL ? a.pnode[n] : 0;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            if (SCH == SCH_AA_LOCAL || d == L::rest) {
+                f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+            } else {
+                const int32_t src = a.map[p - a.off[d]];
+                if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
+                else { f[d] = T(0); miss |= 1u << d; }
+                if (SCH == SCH_AA_GATHER) dst[d] = a.map[p + a.off[d]];
+            }
+        }
+        if (SCH == SCH_AA_GATHER) dst[L::rest] = (int32_t)n;
+        if ((word >> bit) & 1ull) {
+            const int64_t r = a.bpre[n >> 6] + __popcll(word & ((1ull << bit) - 1ull));
+            apply_record<Q, P, SCH>(a, r, n, miss, f);
+        }
+    }
+    emit<Q, P, MODE, SCH>(a, n, f, dst);
+}
+
+// Record/segment kernel.  Records (nodes with a missing link or a boundary
+// condition) take one thread each, their sources from the per-record table
+// rsrc[d][r] (coalesced; no map lookups).  Every other node belongs to a row
+// segment: at most 64 consecutive node ids along the fast axis whose sources
+// in each direction are again consecutive ids, so one wavefront reads each
+// direction as one contiguous run from a base in the segment table.
+//
+// Grid: per XCD (block id mod 8), first its share of record blocks, then its
+// share of segment blocks (4 segments per 256-thread block), so record work
+// starts first and both are spread over all XCDs.
+enum { PART_FUSED = 0, PART_RECORDS = 1, PART_SEGMENTS = 2 };
+
+template <int Q, class P, int MODE, int SCH>
+__device__ __forceinline__ void record_node(const StepArgs& a, int64_t r) {
+    using L = Lat<Q>;
+    using S = typename P::store_t;
+    using T = typename P::calc_t;
+    const S* gin = (const S*)a.gin;
+    const int64_t N = a.n, R = a.nrec;
+    const int64_t n = a.rec_node[r];
+    T f[Q];
+    int32_t dst[Q];
+    uint32_t miss = 0;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        if (SCH == SCH_AA_LOCAL || d == L::rest) {
+            f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+            dst[d] = (int32_t)n;
+        } else {
+            const int32_t src = a.rsrc[(int64_t)d * R + r];
+            if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
+            else { f[d] = T(0); miss |= 1u << d; }
+            // AA gather: the node at x + c_d is the source of direction opp d
+            if (SCH == SCH_AA_GATHER) dst[d] = a.rsrc[(int64_t)L::opp(d) * R + r];
+        }
+    }
+    apply_record<Q, P, SCH>(a, r, n, miss, f);
+    emit<Q, P, MODE, SCH>(a, n, f, dst);
+}
+
+template <int Q, class P, int MODE, int SCH>
+__device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int lane) {
+    using L = Lat<Q>;
+    using S = typename P::store_t;
+    using T = typename P::calc_t;
+    const int32_t* row = a.seg + s * LBM_SEG_W;  // wave-uniform
+    const int32_t start = row[0], len = row[1];
+    if (lane >= len) return;
+    const S* gin = (const S*)a.gin;
+    const int64_t N = a.n;
+    const int64_t n = (int64_t)start + lane;
+    T f[Q];
+    int32_t dst[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        if (SCH == SCH_AA_LOCAL || d == L::rest) {
+            f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+        } else {
+            f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + row[2 + d] + lane);
+        }
+        dst[d] = (SCH == SCH_AA_GATHER && d != L::rest) ? row[2 + L::opp(d)] + lane : (int32_t)n;
+    }
+    emit<Q, P, MODE, SCH>(a, n, f, dst);
+}
+
+template <int Q, class P, int MODE, int SCH, int PART>
+__global__ __launch_bounds__(256) void k_step_rs(StepArgs a) {
+    const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+    const int rb = PART == PART_SEGMENTS ? 0 : a.rb8;
+    if (PART != PART_SEGMENTS && j < rb) {
+        const int64_t r = ((int64_t)xcd * rb + j) * 256 + threadIdx.x;
+        if (r < a.nrec) record_node<Q, P, MODE, SCH>(a, r);
+    } else if (PART != PART_RECORDS) {
+        const int64_t lb = (int64_t)xcd * a.sb8 + (j - rb);
+        const int64_t s = __builtin_amdgcn_readfirstlane((int)(lb * 4 + (threadIdx.x >> 6)));
+        if (s < a.nseg) segment_node<Q, P, MODE, SCH>(a, s, (int)(threadIdx.x & 63));
     }
 }
 
@@ -575,45 +643,54 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a) {
 
 // Every pointer a launch dereferences must be set; a missing one is an error
 // returned to the caller, never a launch.
-inline bool step_args_ok(int mode, int sch, const StepArgs& a) {
-    if (!a.pnode || !a.map || !a.bbits || !a.bpre || !a.gin || !a.faces || !a.waves || !a.iter) return false;
+inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
+    if (!a.gin || !a.faces || !a.waves || !a.iter) return false;
+    if (layout == LBM_LAYOUT_GENERIC && (!a.pnode || !a.map || !a.bbits || !a.bpre)) return false;
+    if (layout != LBM_LAYOUT_GENERIC && ((a.nseg > 0 && !a.seg) || (a.nrec > 0 && (!a.rsrc || !a.rec_node))))
+        return false;
     if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale || !a.bmiss)) return false;
     if (mode != LBM_MODE_PRE && !a.gout) return false;
     if (mode != LBM_MODE_STEP && (!a.fpre || !a.rho || !a.u)) return false;
     if (mode == LBM_MODE_PRE && sch != SCH_TWO) return false;
-    return sch >= SCH_TWO && sch <= SCH_AA_LOCAL;
+    return sch >= SCH_TWO && sch <= SCH_AA_LOCAL && layout >= LBM_LAYOUT_GENERIC && layout <= LBM_LAYOUT_SPLIT;
 }
 
-template <int Q, class P, int MODE, int PART>
-void launch_mode(int sch, int grid, const StepArgs& b, hipStream_t s) {
-    if (sch == SCH_TWO) hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_TWO, PART>), dim3(grid), dim3(256), 0, s, b);
-    else if (sch == SCH_AA_GATHER)
-        hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_AA_GATHER, PART>), dim3(grid), dim3(256), 0, s, b);
-    else hipLaunchKernelGGL((k_step<Q, P, MODE, SCH_AA_LOCAL, PART>), dim3(grid), dim3(256), 0, s, b);
+template <int Q, class P, int MODE, int SCH>
+void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
+    if (layout == LBM_LAYOUT_GENERIC) {
+        hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, b);
+    } else if (layout == LBM_LAYOUT_FUSED) {
+        hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(8 * (b.rb8 + b.sb8)), dim3(256), 0, s, b);
+    } else {
+        // records first, then segments: the two launches write disjoint nodes
+        if (b.rb8 > 0)
+            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_RECORDS>), dim3(8 * b.rb8), dim3(256), 0, s, b);
+        if (b.sb8 > 0)
+            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_SEGMENTS>), dim3(8 * b.sb8), dim3(256), 0, s, b);
+    }
 }
 
-template <int Q, class P, int PART>
-void launch_part(int mode, int sch, int grid, const StepArgs& b, hipStream_t s) {
-    if (mode == LBM_MODE_STEP) launch_mode<Q, P, LBM_MODE_STEP, PART>(sch, grid, b, s);
-    else if (mode == LBM_MODE_SNAP) launch_mode<Q, P, LBM_MODE_SNAP, PART>(sch, grid, b, s);
-    else hipLaunchKernelGGL((k_step<Q, P, LBM_MODE_PRE, SCH_TWO, PART>), dim3(grid), dim3(256), 0, s, b);
+template <int Q, class P, int MODE>
+void launch_sch(int sch, int layout, const StepArgs& b, hipStream_t s) {
+    if (sch == SCH_TWO) launch_layout<Q, P, MODE, SCH_TWO>(layout, b, s);
+    else if (sch == SCH_AA_GATHER) launch_layout<Q, P, MODE, SCH_AA_GATHER>(layout, b, s);
+    else launch_layout<Q, P, MODE, SCH_AA_LOCAL>(layout, b, s);
 }
 
-// split: one launch over the boundary records, one over the interior nodes
-// (the boundary launch goes first: it is small and its tail overlaps the
-// start of the interior launch); otherwise one launch over all nodes.
+// layout: LBM_LAYOUT_GENERIC (one per-node launch through the site map),
+// LBM_LAYOUT_FUSED (records and segments in one launch) or LBM_LAYOUT_SPLIT
+// (the same work in two launches).
 template <int Q, class P>
-hipError_t launch_step_t(int mode, int sch, bool split, const StepArgs& a, hipStream_t s) {
-    if (!step_args_ok(mode, sch, a)) return hipErrorInvalidValue;
-    if (split && a.nrec > 0 && !a.rec_node) return hipErrorInvalidValue;
+hipError_t launch_step_t(int mode, int sch, int layout, const StepArgs& a, hipStream_t s) {
+    if (!step_args_ok(mode, sch, layout, a)) return hipErrorInvalidValue;
     StepArgs b = a;
     b.nblocks = (int)((a.n + 255) / 256);
-    if (!split) {
-        launch_part<Q, P, PART_ALL>(mode, sch, lbm_grid_blocks(a.n, 256), b, s);
-        return hipGetLastError();
-    }
-    if (a.nrec > 0) launch_part<Q, P, PART_BOUNDARY>(mode, sch, lbm_grid_blocks(a.nrec, 256), b, s);
-    launch_part<Q, P, PART_INTERIOR>(mode, sch, lbm_grid_blocks(a.n, 256), b, s);
+    b.rb8 = (int)(((a.nrec + 255) / 256 + 7) / 8);
+    b.sb8 = (int)(((a.nseg + 3) / 4 + 7) / 8);
+    if (layout != LBM_LAYOUT_GENERIC && b.rb8 + b.sb8 == 0) return hipErrorInvalidValue;
+    if (mode == LBM_MODE_STEP) launch_sch<Q, P, LBM_MODE_STEP>(sch, layout, b, s);
+    else if (mode == LBM_MODE_SNAP) launch_sch<Q, P, LBM_MODE_SNAP>(sch, layout, b, s);
+    else launch_layout<Q, P, LBM_MODE_PRE, SCH_TWO>(layout, b, s);
     return hipGetLastError();
 }
 

@@ -68,13 +68,19 @@ def near_eq_state(oracle, q, flags, rng, amp=0.01):
     return f
 
 
+# step kernels: records + segments in one launch, in two, generic per-node kernel
+LAYOUTS = [0, 64, 32]
+
+
 @pytest.mark.parametrize("q", [19, 27])
 @pytest.mark.parametrize("quirks", [1, 0])
-def test_fp64_bitwise_all_faces(P, oracle, ctx, q, quirks):
+@pytest.mark.parametrize("options", LAYOUTS)
+def test_fp64_bitwise_all_faces(P, oracle, ctx, q, quirks, options):
     dims = (10, 11, 12)
     cl, bcs = six_face_box(oracle, dims)
-    print("six-face box q=%d quirks=%d" % (q, quirks), flush=True)
-    dom, ref = both(P, oracle, ctx, q, dims, cl, 0.7, bcs, quirks=quirks)
+    print("six-face box q=%d quirks=%d options=%d" % (q, quirks, options), flush=True)
+    dom, ref = both(P, oracle, ctx, q, dims, cl, 0.7, bcs, quirks=quirks, options=options)
+    assert dom.stats()["layout"] == {0: 1, 64: 2, 32: 0}[options]
     f0 = near_eq_state(oracle, q, cl, np.random.default_rng(q + quirks))
     dom.set_f(f0)
     ref.set_f(f0)
@@ -89,7 +95,8 @@ def test_fp64_bitwise_all_faces(P, oracle, ctx, q, quirks):
     assert bits_equal(dom.wss(), ref.wss())
 
 
-@pytest.mark.parametrize("fast_axis,options", [(-1, 0), (0, 0), (1, 0), (-1, 1), (-1, 2), (-1, 32)])
+@pytest.mark.parametrize("fast_axis,options", [(-1, 0), (0, 0), (1, 0), (-1, 1), (-1, 2), (-1, 32), (-1, 64),
+                                               (0, 32)])
 def test_fp64_bitwise_pipe_config1(P, oracle, ctx, fast_axis, options):
     """D3Q19 FP64, 32x32x128 z-pipe of radius 12, tau 0.8, parabolic inlet
     u_max 0.05, outlet rho 1.0, 2000 steps from rest (the small plain case);
@@ -192,13 +199,18 @@ def test_aa_equals_two_lattice(P, oracle, ctx, storage):
     two = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage)
     aa = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA)
     aa_eager = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA, options=P.capi.OPT_NO_GRAPH)
-    for d in (two, aa, aa_eager):
+    aa_generic = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA,
+                          options=P.capi.OPT_GENERIC_KERNEL)
+    aa_split = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA, options=P.capi.OPT_SPLIT_LAUNCH)
+    others = (aa, aa_eager, aa_generic, aa_split)
+    for d in (two,) + others:
         d.set_waveform(0, P.WAVE_FOURIER, 200.0, a, phi)
     for n in (1, 70, 3, 130):
-        for d in (two, aa, aa_eager):
+        for d in (two,) + others:
             d.step(n)
         f2 = two.f()
-        assert bits_equal(aa.f(), f2) and bits_equal(aa_eager.f(), f2), "after %d" % two.iteration
+        for k, d in enumerate(others):
+            assert bits_equal(d.f(), f2), "AA variant %d after %d" % (k, two.iteration)
         r2, u2 = two.macros()
         r1, u1 = aa.macros()
         assert bits_equal(r1, r2) and bits_equal(u1, u2)
@@ -231,8 +243,8 @@ def test_periodic_shear_wave(P, oracle, ctx, q, scheme):
     ref = oracle.Sim(q, dims, flags, tau, [], quirks=1, periodic=7)
     ref.set_f(f0)
     ref.step(steps)
-    for storage in (0, 1):
-        dom = P.Domain(ctx, dims, flags, tau, [], q=q, storage=storage, scheme=scheme, options=per)
+    for storage, kern in ((0, 0), (0, P.capi.OPT_GENERIC_KERNEL), (1, 0)):
+        dom = P.Domain(ctx, dims, flags, tau, [], q=q, storage=storage, scheme=scheme, options=per | kern)
         dom.set_f(f0)
         dom.step(steps)
         if storage == 0:

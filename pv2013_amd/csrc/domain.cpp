@@ -110,6 +110,10 @@ struct lbm_domain {
     int32_t* rsrc = nullptr;   // record/segment layouts: [d][r] record sources
     int32_t* seg = nullptr;    // [s][LBM_SEG_W] row segments
     int64_t nseg = 0;
+    std::vector<int32_t> wused;      // waveform slots referenced by a boundary condition
+    int32_t* d_wused = nullptr;
+    double* wvals = nullptr;         // [LBM_MAXWAVE][LBM_WCHUNK]
+    int64_t wbase = -1;              // first iteration of the eager chunk in wvals (-1: none)
     uint8_t* btype = nullptr;
     uint32_t* bbb = nullptr;
     uint32_t* bmiss = nullptr;
@@ -249,8 +253,7 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.bprof = D->bprof;
     a.bstale = D->bstale;
     a.faces = D->d_faces;
-    a.waves = D->d_waves;
-    a.iter = D->d_iter;
+    a.wvals = D->wvals;
     a.omega = 1.0 / D->tau;
     for (int d = 0; d < 27; ++d) a.off[d] = D->off[d];
 }
@@ -261,15 +264,13 @@ static int scheme_for(const lbm_domain* D, int64_t it) {
     return (it & 1) == 0 ? 1 : 2;  // AA: gather on even iterations, local on odd
 }
 
-// it_abs = 1: the launch runs iteration it_off; it_abs = 0: iteration
-// (*d_iter + it_off), used inside graphs.
-static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* gout, int64_t it_off,
-                       int it_abs) {
+// wk: the launch's row of the waveform table (its iteration minus the base of
+// the chunk the table holds).
+static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* gout, int wk) {
     lbm_ctx* ctx = D->ctx;
     StepArgs a;
     fill_args(D, a);
-    a.it_off = it_off;
-    a.it_abs = it_abs;
+    a.wk = wk;
     a.gin = gin;
     a.gout = gout;
     a.fpre = D->fpre;
@@ -572,6 +573,14 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->d_faces, D->faces.size()));
     DCHECK(D->alloc(&D->d_waves, D->waves.size()));
     DCHECK(D->alloc(&D->d_iter, 1));
+    for (int b = 0; b < d.nbc; ++b) {
+        const int w = d.bcs[b].waveform;
+        if (d.bcs[b].kind == LBM_BC_VELOCITY && w >= 0 &&
+            std::find(D->wused.begin(), D->wused.end(), w) == D->wused.end())
+            D->wused.push_back(w);
+    }
+    DCHECK(D->alloc(&D->d_wused, std::max<size_t>(D->wused.size(), 1)));
+    DCHECK(D->alloc(&D->wvals, (size_t)LBM_MAXWAVE * LBM_WCHUNK));
     uint8_t *g0 = nullptr, *g1 = nullptr;
     DCHECK(D->alloc(&g0, (size_t)q * D->n * sb));
     if (d.scheme == LBM_SCHEME_TWO_LATTICE) DCHECK(D->alloc(&g1, (size_t)q * D->n * sb));
@@ -603,6 +612,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     }
     if ((rc = upload(D, D->d_faces, D->faces.data(), sizeof(FaceTab) * D->faces.size()))) return fail(rc);
     if ((rc = upload(D, D->d_waves, D->waves.data(), sizeof(WaveTab) * D->waves.size()))) return fail(rc);
+    if ((rc = upload(D, D->d_wused, D->wused.data(), sizeof(int32_t) * D->wused.size()))) return fail(rc);
     if ((rc = upload(D, D->d_wall, D->wall_nodes.data(), sizeof(int32_t) * D->wall_nodes.size()))) return fail(rc);
     if ((rc = upload(D, D->d_bcnode, D->bc_nodes.data(), sizeof(int32_t) * D->bc_nodes.size()))) return fail(rc);
     if ((rc = upload(D, D->d_bcface, D->bc_face.data(), sizeof(int32_t) * D->bc_face.size()))) return fail(rc);
@@ -676,6 +686,7 @@ extern "C" int lbm_set_waveform(lbm_domain* D, int slot, int kind, double period
     }
     CHECK(hipMemcpyAsync(&D->d_waves[slot], &w, sizeof w, hipMemcpyHostToDevice, ctx->stream));
     CHECK(hipStreamSynchronize(ctx->stream));
+    D->wbase = -1;  // tabulated values are stale
     // the two-lattice read-back recomputes the last step with the waveform
     // table; an AA snapshot was written by that step and stays valid
     if (D->scheme == LBM_SCHEME_TWO_LATTICE) D->pre_iter = -1;
@@ -689,10 +700,15 @@ extern "C" int lbm_set_waveform(lbm_domain* D, int slot, int kind, double period
 // Nothing captured uses a memset node.
 static int build_graph(lbm_domain* D, int c) {
     lbm_ctx* ctx = D->ctx;
+    static_assert(lbm_domain::GRAPH_STEPS == LBM_WCHUNK, "one waveform chunk per graph replay");
     CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = LBM_OK;
+    // the replay's waveform values, from the device-resident iteration base
+    if (launch_wave_chunk(D->d_waves, D->d_wused, (int)D->wused.size(), D->d_iter, 0, 0, D->wvals, ctx->stream) !=
+        hipSuccess)
+        rc = LBM_EIO;
     for (int i = 0; i < lbm_domain::GRAPH_STEPS && rc == LBM_OK; ++i)
-        rc = launch_step(D, LBM_MODE_STEP, scheme_for(D, i), D->g[(c + i) & 1], D->g[(c + i + 1) & 1], i, 0);
+        rc = launch_step(D, LBM_MODE_STEP, scheme_for(D, i), D->g[(c + i) & 1], D->g[(c + i + 1) & 1], i);
     if (rc == LBM_OK && launch_advance_iter(D->d_iter, lbm_domain::GRAPH_STEPS, ctx->stream) != hipSuccess)
         rc = LBM_EIO;
     hipGraph_t g = nullptr;
@@ -707,8 +723,26 @@ static int build_graph(lbm_domain* D, int c) {
     return LBM_OK;
 }
 
+// Row of the waveform table for an eager launch at iteration it, tabulating a
+// new chunk from it when the table does not hold it.
+static int eager_wave_row(lbm_domain* D, int64_t it, int* wk) {
+    lbm_ctx* ctx = D->ctx;
+    *wk = 0;
+    if (D->wused.empty()) return LBM_OK;
+    if (D->wbase < 0 || it < D->wbase || it >= D->wbase + LBM_WCHUNK) {
+        CHECK(launch_wave_chunk(D->d_waves, D->d_wused, (int)D->wused.size(), nullptr, it, 1, D->wvals,
+                                ctx->stream));
+        D->wbase = it;
+    }
+    *wk = (int)(it - D->wbase);
+    return LBM_OK;
+}
+
 static int eager_step(lbm_domain* D, int mode) {
-    int rc = launch_step(D, mode, scheme_for(D, D->iter), D->g[D->cur], D->g[D->cur ^ 1], D->iter, 1);
+    int wk = 0;
+    int rc = eager_wave_row(D, D->iter, &wk);
+    if (rc) return rc;
+    rc = launch_step(D, mode, scheme_for(D, D->iter), D->g[D->cur], D->g[D->cur ^ 1], wk);
     if (rc) return rc;
     if (D->scheme == LBM_SCHEME_TWO_LATTICE) D->cur ^= 1;
     D->iter++;
@@ -737,6 +771,7 @@ static int step_n(lbm_domain* D, int64_t nsteps) {
                 if (rc) return rc;
             }
             CHECK(hipGraphLaunch(D->exec[c], ctx->stream));
+            D->wbase = -1;  // the replay re-tabulated wvals for its own chunk
             D->iter += G;  // G is even: the lattice parity is unchanged
             body -= G;
         }
@@ -847,7 +882,9 @@ static int ensure_pre(lbm_domain* D) {
     const unsigned grid = (unsigned)((D->n + 255) / 256);
     if (D->state == 2) {
         if (D->scheme == LBM_SCHEME_AA) return lbm_ctx_fail(ctx, LBM_EIO, "AA state snapshot missing");
-        rc = launch_step(D, LBM_MODE_PRE, 0, D->g[D->cur ^ 1], nullptr, D->iter - 1, 1);
+        int wk = 0;
+        if ((rc = eager_wave_row(D, D->iter - 1, &wk))) return rc;
+        rc = launch_step(D, LBM_MODE_PRE, 0, D->g[D->cur ^ 1], nullptr, wk);
         if (rc) return rc;
     } else {
         if (D->state == 0) hipLaunchKernelGGL(k_fill_rest, dim3(grid), dim3(256), 0, ctx->stream, D->fpre, D->n, D->q);

@@ -23,6 +23,23 @@ hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s) {
     return hipGetLastError();
 }
 
+// One block per used waveform slot, one thread per iteration of the chunk.
+__global__ void k_wave_chunk(const WaveTab* waves, const int32_t* used, const int64_t* iter, int64_t it0, int it_abs,
+                             double* wvals) {
+    const int slot = used[blockIdx.x];
+    const int k = threadIdx.x;
+    const int64_t it = (it_abs ? it0 : *iter) + k;
+    wvals[slot * LBM_WCHUNK + k] = wave_value(waves[slot], it);
+}
+
+hipError_t launch_wave_chunk(const WaveTab* waves, const int32_t* used, int nused, const int64_t* iter,
+                             int64_t it0, int it_abs, double* wvals, hipStream_t s) {
+    if (nused <= 0) return hipSuccess;
+    if (!waves || !used || !wvals || (!it_abs && !iter)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_wave_chunk, dim3(nused), dim3(LBM_WCHUNK), 0, s, waves, used, iter, it0, it_abs, wvals);
+    return hipGetLastError();
+}
+
 hipError_t launch_step_f64(int mode, int sch, int layout, const StepArgs& a, hipStream_t s) {
     if (a.q == 19) return launch_step_t<19, PrecF64>(mode, sch, layout, a, s);
     return launch_step_t<27, PrecF64>(mode, sch, layout, a, s);

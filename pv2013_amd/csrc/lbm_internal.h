@@ -53,10 +53,8 @@ struct StepArgs {
     const double* bprof;    // per record: velocity profile factor
     const void* bstale;     // [e][r] stale populations (double, or float deviations)
     const FaceTab* faces;
-    const WaveTab* waves;
-    int64_t* iter;          // device iteration base (graph replays)
-    int64_t it_off;         // iteration of this launch: it_off, or *iter + it_off
-    int32_t it_abs;         //   when it_abs is 0 (graph nodes read the base)
+    const double* wvals;    // [slot][LBM_WCHUNK] waveform values of the current chunk
+    int32_t wk;             // this launch's row in the chunk
     const void* gin;        // [d][n] post-collision populations
     void* gout;
     double omega;
@@ -98,9 +96,15 @@ enum { LBM_LAYOUT_GENERIC = 0, LBM_LAYOUT_FUSED = 1, LBM_LAYOUT_SPLIT = 2 };
 // direction d from node base[d] + l.
 constexpr int LBM_SEG_W = 32;
 
+// Waveform values are tabulated once per chunk of LBM_WCHUNK consecutive
+// iterations: wvals[slot][k] = W_slot(base + k), base = it0 (it_abs) or *iter.
+constexpr int LBM_WCHUNK = 64;
+
 // kernels_f64.hip
 hipError_t launch_advance_iter(int64_t* iter, int64_t by, hipStream_t s);
 hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s);
+hipError_t launch_wave_chunk(const WaveTab* waves, const int32_t* used, int nused, const int64_t* iter,
+                             int64_t it0, int it_abs, double* wvals, hipStream_t s);
 // sch: 0 two-lattice pull, 1 AA gather step, 2 AA local step
 // layout: LBM_LAYOUT_*
 hipError_t launch_step_f64(int mode, int sch, int layout, const StepArgs& a, hipStream_t s);

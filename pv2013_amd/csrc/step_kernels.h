@@ -413,11 +413,7 @@ __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64
         const FaceTab& F = a.faces[type - 3];
         double uv[3] = {0.0, 0.0, 0.0};
         if (F.kind == LBM_BC_VELOCITY) {
-            double w = 1.0;
-            if (F.waveform >= 0) {
-                const int64_t it = a.it_abs ? a.it_off : *a.iter + a.it_off;
-                w = wave_value(a.waves[F.waveform], it);
-            }
+            const double w = F.waveform >= 0 ? a.wvals[F.waveform * LBM_WCHUNK + a.wk] : 1.0;
             const double fac = a.bprof[r] * w;
             uv[0] = F.u[0] * fac; uv[1] = F.u[1] * fac; uv[2] = F.u[2] * fac;
         }
@@ -426,9 +422,10 @@ __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64
 }
 
 // Pre-state output (modes PRE / SNAP), collision and store of node n.
-// dst (AA gather only): destination node of each population, -1 if none.
-template <int Q, class P, int MODE, int SCH>
-__device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::calc_t* f, const int32_t* dst) {
+// dst(d) (AA gather only): destination node of population d, -1 if none;
+// evaluated at the store so no per-direction index stays live in registers.
+template <int Q, class P, int MODE, int SCH, class Dst>
+__device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::calc_t* f, Dst dst) {
     using L = Lat<Q>;
     using S = typename P::store_t;
     using T = typename P::calc_t;
@@ -451,7 +448,8 @@ __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::c
         } else if (SCH == SCH_AA_LOCAL) {
             gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
         } else {
-            if (dst[d] >= 0) gout[(int64_t)d * N + dst[d]] = (S)g[d];
+            const int32_t t = dst(d);
+            if (t >= 0) gout[(int64_t)d * N + t] = (S)g[d];
             else gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
         }
     }
@@ -532,7 +530,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
             apply_record<Q, P, SCH>(a, r, n, miss, f);
         }
     }
-    emit<Q, P, MODE, SCH>(a, n, f, dst);
+    emit<Q, P, MODE, SCH>(a, n, f, [&](int d) { return dst[d]; });
 }
 
 // Record/segment kernel.  Records (nodes with a missing link or a boundary
@@ -556,23 +554,22 @@ __device__ __forceinline__ void record_node(const StepArgs& a, int64_t r) {
     const int64_t N = a.n, R = a.nrec;
     const int64_t n = a.rec_node[r];
     T f[Q];
-    int32_t dst[Q];
     uint32_t miss = 0;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         if (SCH == SCH_AA_LOCAL || d == L::rest) {
             f[d] = load_pop<P>(gin, (int64_t)d * N + n);
-            dst[d] = (int32_t)n;
         } else {
             const int32_t src = a.rsrc[(int64_t)d * R + r];
             if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
             else { f[d] = T(0); miss |= 1u << d; }
-            // AA gather: the node at x + c_d is the source of direction opp d
-            if (SCH == SCH_AA_GATHER) dst[d] = a.rsrc[(int64_t)L::opp(d) * R + r];
         }
     }
     apply_record<Q, P, SCH>(a, r, n, miss, f);
-    emit<Q, P, MODE, SCH>(a, n, f, dst);
+    // AA gather: the node at x + c_d is the source of direction opp d
+    emit<Q, P, MODE, SCH>(a, n, f, [&](int d) {
+        return d == L::rest ? (int32_t)n : a.rsrc[(int64_t)L::opp(d) * R + r];
+    });
 }
 
 template <int Q, class P, int MODE, int SCH>
@@ -587,7 +584,6 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
     const int64_t N = a.n;
     const int64_t n = (int64_t)start + lane;
     T f[Q];
-    int32_t dst[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         if (SCH == SCH_AA_LOCAL || d == L::rest) {
@@ -595,9 +591,8 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
         } else {
             f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + row[2 + d] + lane);
         }
-        dst[d] = (SCH == SCH_AA_GATHER && d != L::rest) ? row[2 + L::opp(d)] + lane : (int32_t)n;
     }
-    emit<Q, P, MODE, SCH>(a, n, f, dst);
+    emit<Q, P, MODE, SCH>(a, n, f, [&](int d) { return d == L::rest ? (int32_t)n : row[2 + L::opp(d)] + lane; });
 }
 
 template <int Q, class P, int MODE, int SCH, int PART>
@@ -644,7 +639,7 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a) {
 // Every pointer a launch dereferences must be set; a missing one is an error
 // returned to the caller, never a launch.
 inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
-    if (!a.gin || !a.faces || !a.waves || !a.iter) return false;
+    if (!a.gin || !a.faces || !a.wvals || a.wk < 0 || a.wk >= LBM_WCHUNK) return false;
     if (layout == LBM_LAYOUT_GENERIC && (!a.pnode || !a.map || !a.bbits || !a.bpre)) return false;
     if (layout != LBM_LAYOUT_GENERIC && ((a.nseg > 0 && !a.seg) || (a.nrec > 0 && (!a.rsrc || !a.rec_node))))
         return false;

@@ -199,6 +199,21 @@ __device__ void zouhe_f64(const FaceTab& F, const double* uv, bool typo, double*
 }
 
 // ----------------------------------------------------- FP32 on shifted values
+//
+// Constants such as the lattice weights and omega are applied as a float pair
+// (hi, lo), hi = (float)c, lo = (float)(c - hi), through one fused
+// multiply-add: x*c = fma(x, hi, x*lo).  A plain float constant carries a
+// relative error of up to 6e-8 that is the same at every node and every step:
+// (1/36)f and (1/18)f are both 7.45e-9 too large, so the equilibrium would hold
+// (1 + 7.45e-9) times the node's momentum and every collision would add
+// omega * 7.45e-9 of it, a bias that compounds over the run (about 1e-5 of
+// the velocity after a few thousand steps).  With the pair the constant is
+// exact to about 2^-48 and only unbiased per-node rounding remains.
+struct CPair {
+    float hi, lo;
+};
+__host__ __device__ constexpr CPair cpair(double c) { return {(float)c, (float)(c - (double)(float)c)}; }
+__device__ __forceinline__ float mulc(float x, CPair c) { return __builtin_fmaf(x, c.hi, x * c.lo); }
 
 // Moments of the deviations f - w: rho = 1 + sum dev, j = sum c dev.
 template <int Q>
@@ -220,7 +235,7 @@ __device__ __forceinline__ void moments_dev(const float* fd, float& drho, float*
 // BGK on deviations: g - w = (f - w) + omega ((f_eq - w) - (f - w)),
 // f_eq - w = w (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).
 template <int Q>
-__device__ __forceinline__ void collide_dev(float* fd, float omega) {
+__device__ __forceinline__ void collide_dev(float* fd, CPair omega) {
     using L = Lat<Q>;
     float drho, u[3];
     moments_dev<Q>(fd, drho, u);
@@ -232,8 +247,8 @@ __device__ __forceinline__ void collide_dev(float* fd, float omega) {
         if (L::c(d, 0)) cu += L::c(d, 0) > 0 ? u[0] : -u[0];
         if (L::c(d, 1)) cu += L::c(d, 1) > 0 ? u[1] : -u[1];
         if (L::c(d, 2)) cu += L::c(d, 2) > 0 ? u[2] : -u[2];
-        const float feqd = (float)L::w(d) * (drho + rho * (cu * (3.f + 4.5f * cu) - usq15));
-        fd[d] = fd[d] + omega * (feqd - fd[d]);
+        const float feqd = mulc(drho + rho * (cu * (3.f + 4.5f * cu) - usq15), cpair(L::w(d)));
+        fd[d] = fd[d] + mulc(feqd - fd[d], omega);
     }
 }
 
@@ -275,9 +290,9 @@ __device__ void zouhe_dev(const FaceTab& F, const float* uv, bool typo, float* f
         if (Q == 19) {
             if (AX == 0 && SG == 1 && t == 2 && typo && F.kind == LBM_BC_PRESSURE)
                 PM += fd[10] - fd[16];  // Simulation.cpp:1124-1129
-            N[t] = 0.5f * PM - (1.f / 3.f) * rho * u[t];
+            N[t] = 0.5f * PM - mulc(rho * u[t], cpair(1.0 / 3.0));
         } else {
-            N[t] = (1.f / 6.f) * PM - (1.f / 9.f) * rho * u[t];
+            N[t] = mulc(PM, cpair(1.0 / 6.0)) - mulc(rho * u[t], cpair(1.0 / 9.0));
         }
     }
     float g[Q];
@@ -290,10 +305,10 @@ __device__ void zouhe_dev(const FaceTab& F, const float* uv, bool typo, float* f
         int nt = 0;
         if (L::c(d, T0)) { cu += L::c(d, T0) > 0 ? u[T0] : -u[T0]; ++nt; }
         if (L::c(d, T1)) { cu += L::c(d, T1) > 0 ? u[T1] : -u[T1]; ++nt; }
-        float coef;
-        if (Q == 19) coef = nt == 0 ? 1.f / 3.f : 1.f / 6.f;
-        else coef = nt == 0 ? 4.f / 9.f : (nt == 1 ? 1.f / 9.f : 1.f / 36.f);
-        float v = g[L::opp(d)] + coef * rho * cu;
+        double coef;
+        if (Q == 19) coef = nt == 0 ? 1.0 / 3.0 : 1.0 / 6.0;
+        else coef = nt == 0 ? 4.0 / 9.0 : (nt == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
+        float v = g[L::opp(d)] + mulc(rho * cu, cpair(coef));
         if (L::c(d, T0)) v += L::c(d, T0) > 0 ? -N[T0] : N[T0];
         if (L::c(d, T1)) v += L::c(d, T1) > 0 ? -N[T1] : N[T1];
         fd[d] = v;
@@ -439,7 +454,7 @@ __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::c
     } else {
 #pragma unroll
         for (int d = 0; d < Q; ++d) g[d] = f[d];
-        collide_dev<Q>((float*)g, (float)a.omega);
+        collide_dev<Q>((float*)g, cpair(a.omega));
     }
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
@@ -630,7 +645,7 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a) {
     } else {
 #pragma unroll
         for (int d = 0; d < Q; ++d) g[d] = a.fpre ? (float)(a.fpre[(int64_t)d * a.n + n] - L::w(d)) : 0.f;
-        collide_dev<Q>((float*)g, (float)a.omega);
+        collide_dev<Q>((float*)g, cpair(a.omega));
     }
 #pragma unroll
     for (int d = 0; d < Q; ++d) gout[(int64_t)(SWAP ? L::opp(d) : d) * a.n + n] = (S)g[d];

@@ -88,11 +88,13 @@ def cpu_baseline(case, budget_s=12.0):
                       "FP64 oracle, 1 thread" % (k, sim.nodes)}
 
 
-def traffic_from_profiles(storage_name):
+def traffic_from_profiles(key):
+    """HBM bytes per step measured with rocprofv3 FETCH_SIZE / WRITE_SIZE for
+    this storage / scheme / layout (profiles/step_traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "step_traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(storage_name)
+            return json.load(fh).get(key)
     except (OSError, ValueError):
         return None
 
@@ -106,7 +108,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64 / FP16 comparison runs and the accuracy protocol")
-    ap.add_argument("--scheme", default="two", choices=["two", "aa"])
+    ap.add_argument("--scheme", default="aa", choices=["two", "aa"],
+                    help="streaming: AA (one lattice, default) or two-lattice pull; identical results")
     ap.add_argument("--layout", default="fused", choices=["fused", "split", "generic"],
                     help="step kernel: records + row segments in one launch (default) or two, "
                          "or the generic per-node kernel")
@@ -179,7 +182,8 @@ def main():
                    "scheme": "AA" if args.scheme == "aa" else "two-lattice pull",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(args.storage)},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles("%s_%s_%s" % (args.storage, args.scheme, args.layout))},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -110,6 +110,8 @@ struct lbm_domain {
     int32_t* rsrc = nullptr;   // record/segment layouts: [d][r] record sources
     int32_t* seg = nullptr;    // [s][LBM_SEG_W] row segments
     int64_t nseg = 0;
+    int32_t* wlist = nullptr;  // wavefront work list (records and segments in node order)
+    int64_t nwaves = 0;
     std::vector<int32_t> wused;      // waveform slots referenced by a boundary condition
     int32_t* d_wused = nullptr;
     double* wvals = nullptr;         // [LBM_MAXWAVE][LBM_WCHUNK]
@@ -250,6 +252,8 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.rsrc = D->rsrc;
     a.seg = D->seg;
     a.nseg = D->nseg;
+    a.wlist = D->wlist;
+    a.nwaves = D->nwaves;
     a.bprof = D->bprof;
     a.bstale = D->bstale;
     a.faces = D->d_faces;
@@ -497,7 +501,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     // last one's source; segments also end at multiples of 64 node ids so a
     // wavefront's stores cover whole cache lines.  Every run this accepts is
     // contiguous by construction, and the bounds are checked again below.
-    std::vector<int32_t> rsrc, segs;
+    std::vector<int32_t> rsrc, segs, wlist;
     if (D->layout != LBM_LAYOUT_GENERIC) {
         rsrc.assign((size_t)q * std::max<int64_t>(D->nrec, 1), -1);
         for (int64_t r = 0; r < D->nrec; ++r) {
@@ -535,6 +539,16 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             }
         }
         if (D->nseg == 0) segs.assign(LBM_SEG_W, 0);
+        // merge segments and 64-record chunks by first node id
+        const int64_t nchunk = (D->nrec + 63) / 64;
+        wlist.reserve((size_t)(D->nseg + nchunk));
+        int64_t si = 0, ci = 0;
+        while (si < D->nseg || ci < nchunk) {
+            const bool take_seg = ci >= nchunk || (si < D->nseg && segs[si * LBM_SEG_W] < rec_node[ci * 64]);
+            if (take_seg) wlist.push_back((int32_t)si++);
+            else wlist.push_back((int32_t)(-(ci++) - 1));
+        }
+        D->nwaves = (int64_t)wlist.size();
     }
     // device allocation
     int rc = LBM_OK;
@@ -566,6 +580,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if (D->layout != LBM_LAYOUT_GENERIC) {
         DCHECK(D->alloc(&D->rsrc, rsrc.size()));
         DCHECK(D->alloc(&D->seg, segs.size()));
+        DCHECK(D->alloc(&D->wlist, std::max<size_t>(wlist.size(), 1)));
     }
     uint8_t* stale_raw = nullptr;
     DCHECK(D->alloc(&stale_raw, (size_t)q * std::max<int64_t>(D->nrec, 1) * stale_sb));
@@ -609,6 +624,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if (D->layout != LBM_LAYOUT_GENERIC) {
         if ((rc = upload(D, D->rsrc, rsrc.data(), sizeof(int32_t) * rsrc.size()))) return fail(rc);
         if ((rc = upload(D, D->seg, segs.data(), sizeof(int32_t) * segs.size()))) return fail(rc);
+        if ((rc = upload(D, D->wlist, wlist.data(), sizeof(int32_t) * wlist.size()))) return fail(rc);
     }
     if ((rc = upload(D, D->d_faces, D->faces.data(), sizeof(FaceTab) * D->faces.size()))) return fail(rc);
     if ((rc = upload(D, D->d_waves, D->waves.data(), sizeof(WaveTab) * D->waves.size()))) return fail(rc);

@@ -49,7 +49,9 @@ struct StepArgs {
     const int32_t* rsrc;    // [d][r] source node of record r in direction d (x - c_d), -1: none
     const int32_t* seg;     // [s][LBM_SEG_W] row segments: start, length, source base per direction
     int64_t nseg;
-    int32_t rb8, sb8;       // record / segment blocks per XCD (set by the launcher)
+    const int32_t* wlist;   // wavefront work list in node order: s >= 0 segment s, -(k+1) records 64k..64k+63
+    int64_t nwaves;
+    int32_t rb8, sb8;       // record / segment blocks per XCD (two-launch layout, set by the launcher)
     const double* bprof;    // per record: velocity profile factor
     const void* bstale;     // [e][r] stale populations (double, or float deviations)
     const FaceTab* faces;

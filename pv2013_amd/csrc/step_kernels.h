@@ -555,9 +555,13 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 // in each direction are again consecutive ids, so one wavefront reads each
 // direction as one contiguous run from a base in the segment table.
 //
-// Grid: per XCD (block id mod 8), first its share of record blocks, then its
-// share of segment blocks (4 segments per 256-thread block), so record work
-// starts first and both are spread over all XCDs.
+// One launch (PART_FUSED): each 64-lane wavefront takes one entry of a work
+// list ordered by first node id, either a segment or 64 consecutive records,
+// so records and the segments around them (which share cache lines of every
+// population array) run at the same time on the same XCD.  Blocks map to XCDs
+// in contiguous ranges (logical_block).  Two launches (PART_RECORDS, then
+// PART_SEGMENTS): per XCD (block id mod 8) a contiguous share of record blocks
+// or of segment blocks (4 segments per 256-thread block).
 enum { PART_FUSED = 0, PART_RECORDS = 1, PART_SEGMENTS = 2 };
 
 template <int Q, class P, int MODE, int SCH>
@@ -612,6 +616,21 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
 
 template <int Q, class P, int MODE, int SCH, int PART>
 __global__ __launch_bounds__(256) void k_step_rs(StepArgs a) {
+    if constexpr (PART == PART_FUSED) {
+        const int64_t lb = logical_block();
+        if (lb >= a.nblocks) return;
+        const int64_t w = __builtin_amdgcn_readfirstlane((int)(lb * 4 + (threadIdx.x >> 6)));
+        if (w >= a.nwaves) return;
+        const int32_t e = a.wlist[w];  // wave-uniform
+        const int lane = (int)(threadIdx.x & 63);
+        if (e >= 0) {
+            segment_node<Q, P, MODE, SCH>(a, e, lane);
+        } else {
+            const int64_t r = (int64_t)(-(e + 1)) * 64 + lane;
+            if (r < a.nrec) record_node<Q, P, MODE, SCH>(a, r);
+        }
+        return;
+    }
     const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
     const int rb = PART == PART_SEGMENTS ? 0 : a.rb8;
     if (PART != PART_SEGMENTS && j < rb) {
@@ -658,6 +677,7 @@ inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
     if (layout == LBM_LAYOUT_GENERIC && (!a.pnode || !a.map || !a.bbits || !a.bpre)) return false;
     if (layout != LBM_LAYOUT_GENERIC && ((a.nseg > 0 && !a.seg) || (a.nrec > 0 && (!a.rsrc || !a.rec_node))))
         return false;
+    if (layout == LBM_LAYOUT_FUSED && (!a.wlist || a.nwaves != a.nseg + (a.nrec + 63) / 64)) return false;
     if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale || !a.bmiss)) return false;
     if (mode != LBM_MODE_PRE && !a.gout) return false;
     if (mode != LBM_MODE_STEP && (!a.fpre || !a.rho || !a.u)) return false;
@@ -670,7 +690,8 @@ void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
     if (layout == LBM_LAYOUT_GENERIC) {
         hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, b);
     } else if (layout == LBM_LAYOUT_FUSED) {
-        hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(8 * (b.rb8 + b.sb8)), dim3(256), 0, s, b);
+        hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(b.nwaves, 4)), dim3(256), 0,
+                           s, b);
     } else {
         // records first, then segments: the two launches write disjoint nodes
         if (b.rb8 > 0)
@@ -694,7 +715,7 @@ template <int Q, class P>
 hipError_t launch_step_t(int mode, int sch, int layout, const StepArgs& a, hipStream_t s) {
     if (!step_args_ok(mode, sch, layout, a)) return hipErrorInvalidValue;
     StepArgs b = a;
-    b.nblocks = (int)((a.n + 255) / 256);
+    b.nblocks = (int)(layout == LBM_LAYOUT_FUSED ? (a.nwaves + 3) / 4 : (a.n + 255) / 256);
     b.rb8 = (int)(((a.nrec + 255) / 256 + 7) / 8);
     b.sb8 = (int)(((a.nseg + 3) / 4 + 7) / 8);
     if (layout != LBM_LAYOUT_GENERIC && b.rb8 + b.sb8 == 0) return hipErrorInvalidValue;

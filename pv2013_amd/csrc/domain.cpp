@@ -94,6 +94,7 @@ struct lbm_domain {
     int perm[3] = {0, 1, 2};  // perm[0] is the fastest-varying axis of the node order
     int pd[3] = {0, 0, 0};    // padded dims in permuted order
     int64_t nsites = 0, n = 0, nrec = 0;
+    int64_t ns = 0;  // population array stride: n rounded up to 64 (arrays 64-node aligned, pairs share words)
     std::vector<int64_t> node_site;  // host: node -> x-fastest site
     std::vector<int32_t> wall_nodes;
     std::vector<int32_t> bc_nodes, bc_face;
@@ -221,14 +222,24 @@ static int upload(lbm_domain* D, void* dst, const void* src, size_t bytes) {
     return LBM_OK;
 }
 
-// Stale populations of boundary records, [e][r] in the storage's calc type.
+// Stale populations of boundary records, [e][r] in the storage type (FP32 and
+// FP16 storage hold f - w, rounded through FP32 as the kernels round).
 static int upload_stale(lbm_domain* D, const std::vector<double>& stale_f64) {
     if (D->storage == LBM_STORE_F64) return upload(D, D->bstale, stale_f64.data(), sizeof(double) * stale_f64.size());
-    std::vector<float> dev(stale_f64.size());
-    for (int e = 0; e < D->q; ++e)
-        for (int64_t r = 0; r < D->nrec; ++r)
-            dev[e * D->nrec + r] = (float)(stale_f64[e * D->nrec + r] - lat_w(D->q, e));
-    int rc = upload(D, D->bstale, dev.data(), sizeof(float) * dev.size());
+    int rc;
+    if (D->storage == LBM_STORE_F32) {
+        std::vector<float> dev(stale_f64.size());
+        for (int e = 0; e < D->q; ++e)
+            for (int64_t r = 0; r < D->nrec; ++r)
+                dev[e * D->nrec + r] = (float)(stale_f64[e * D->nrec + r] - lat_w(D->q, e));
+        rc = upload(D, D->bstale, dev.data(), sizeof(float) * dev.size());
+    } else {
+        std::vector<_Float16> dev(stale_f64.size());
+        for (int e = 0; e < D->q; ++e)
+            for (int64_t r = 0; r < D->nrec; ++r)
+                dev[e * D->nrec + r] = (_Float16)(float)(stale_f64[e * D->nrec + r] - lat_w(D->q, e));
+        rc = upload(D, D->bstale, dev.data(), sizeof(_Float16) * dev.size());
+    }
     if (rc) return rc;
     lbm_ctx* ctx = D->ctx;
     CHECK(hipStreamSynchronize(ctx->stream));
@@ -238,6 +249,7 @@ static int upload_stale(lbm_domain* D, const std::vector<double>& stale_f64) {
 static void fill_args(const lbm_domain* D, StepArgs& a) {
     std::memset(&a, 0, sizeof a);
     a.n = D->n;
+    a.ns = D->ns;
     a.nrec = D->nrec;
     a.q = D->q;
     a.pnode = D->pnode;
@@ -291,6 +303,7 @@ static int init_from(lbm_domain* D, const double* fpre_dev) {
     InitArgs a;
     std::memset(&a, 0, sizeof a);
     a.n = D->n;
+    a.ns = D->ns;
     a.fpre = fpre_dev;
     a.gout = D->g[D->cur];
     a.omega = 1.0 / D->tau;
@@ -432,12 +445,16 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     for (int b = 0; b < d.nbc; ++b) D->faces[b] = make_face(d.bcs[b]);
     D->waves.assign(LBM_MAXWAVE, WaveTab());
     for (auto& w : D->waves) std::memset(&w, 0, sizeof w);
+    D->layout = (d.options & (LBM_OPT_GENERIC_KERNEL | LBM_OPT_NO_ROW_FASTPATH))
+                    ? LBM_LAYOUT_GENERIC
+                    : ((d.options & LBM_OPT_SPLIT_LAUNCH) ? LBM_LAYOUT_SPLIT : LBM_LAYOUT_FUSED);
+    // missing links of every node
+    std::vector<uint32_t> missv((size_t)D->n, 0u);
     for (int64_t m = 0; m < D->n; ++m) {
         const int64_t st = site[m];
         ijk[0] = (int)(st % nx);
         ijk[1] = (int)((st / nx) % ny);
         ijk[2] = (int)(st / ((int64_t)nx * ny));
-        const int type = flags[st];
         uint32_t miss = 0;
         for (int e = 0; e < q; ++e) {
             if (e == rest) continue;
@@ -448,12 +465,24 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             const bool in = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
             if (!in || !flags[x + (int64_t)nx * (y + (int64_t)ny * z)]) miss |= 1u << e;
         }
+        missv[m] = miss;
+    }
+    // Nodes with a boundary record: a missing link or a boundary condition.
+    std::vector<uint8_t> isrec((size_t)D->n, 0);
+    for (int64_t m = 0; m < D->n; ++m) isrec[m] = flags[site[m]] >= 3 || missv[m] != 0;
+    for (int64_t m = 0; m < D->n; ++m) {
+        const int64_t st = site[m];
+        ijk[0] = (int)(st % nx);
+        ijk[1] = (int)((st / nx) % ny);
+        ijk[2] = (int)(st / ((int64_t)nx * ny));
+        const int type = flags[st];
+        const uint32_t miss = missv[m];
         if (type == 2) D->wall_nodes.push_back((int32_t)m);
         if (type >= 3) {
             D->bc_nodes.push_back((int32_t)m);
             D->bc_face.push_back(type - 3);
         }
-        if (type < 3 && miss == 0) continue;
+        if (!isrec[m]) continue;
         bits[m >> 6] |= 1ull << (m & 63);
         uint32_t bb;
         if (D->quirks) {
@@ -479,9 +508,6 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     // Wavefronts (64 consecutive node ids) made of interior nodes only, all in
     // one row at consecutive positions along the fast axis: then for every
     // direction the 64 sources are fluid, hence one contiguous run of ids.
-    D->layout = (d.options & (LBM_OPT_GENERIC_KERNEL | LBM_OPT_NO_ROW_FASTPATH))
-                    ? LBM_LAYOUT_GENERIC
-                    : ((d.options & LBM_OPT_SPLIT_LAUNCH) ? LBM_LAYOUT_SPLIT : LBM_LAYOUT_FUSED);
     const bool fastpath = D->layout == LBM_LAYOUT_GENERIC && !(d.options & LBM_OPT_NO_ROW_FASTPATH);
     std::vector<uint32_t> wreg((size_t)((nwords + 31) / 32), 0u);
     if (fastpath) {
@@ -566,7 +592,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
         }                                                                                            \
     } while (0)
     const size_t sb = store_bytes(d.storage);
-    const size_t stale_sb = d.storage == LBM_STORE_F64 ? 8 : 4;
+    const size_t stale_sb = store_bytes(d.storage);
     DCHECK(D->alloc(&D->pnode, (size_t)D->n));
     DCHECK(D->alloc(&D->map, (size_t)mapn));
     DCHECK(D->alloc(&D->bbits, (size_t)nwords));
@@ -597,8 +623,9 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->d_wused, std::max<size_t>(D->wused.size(), 1)));
     DCHECK(D->alloc(&D->wvals, (size_t)LBM_MAXWAVE * LBM_WCHUNK));
     uint8_t *g0 = nullptr, *g1 = nullptr;
-    DCHECK(D->alloc(&g0, (size_t)q * D->n * sb));
-    if (d.scheme == LBM_SCHEME_TWO_LATTICE) DCHECK(D->alloc(&g1, (size_t)q * D->n * sb));
+    D->ns = (D->n + 63) / 64 * 64;
+    DCHECK(D->alloc(&g0, (size_t)q * D->ns * sb));
+    if (d.scheme == LBM_SCHEME_TWO_LATTICE) DCHECK(D->alloc(&g1, (size_t)q * D->ns * sb));
     D->g[0] = g0;
     D->g[1] = d.scheme == LBM_SCHEME_TWO_LATTICE ? g1 : g0;  // AA: one lattice, updated in place
     if (d.scheme == LBM_SCHEME_AA) {

@@ -33,6 +33,7 @@ struct WaveTab {
 // packed node id n; boundary arrays by the boundary rank r of n.
 struct StepArgs {
     int64_t n;              // fluid nodes
+    int64_t ns;             // stride of the population arrays (n rounded up to 64)
     int64_t nrec;           // boundary records
     int32_t nblocks;        // logical blocks (grid is rounded up to a multiple of 8)
     int32_t q;
@@ -57,7 +58,7 @@ struct StepArgs {
     const FaceTab* faces;
     const double* wvals;    // [slot][LBM_WCHUNK] waveform values of the current chunk
     int32_t wk;             // this launch's row in the chunk
-    const void* gin;        // [d][n] post-collision populations
+    const void* gin;        // [d][ns] post-collision populations
     void* gout;
     double omega;
     int32_t off[27];        // padded map offset of c_d
@@ -69,6 +70,7 @@ struct StepArgs {
 
 struct InitArgs {
     int64_t n;
+    int64_t ns;          // population array stride
     int32_t nblocks;
     const double* fpre;  // [d][n]
     void* gout;

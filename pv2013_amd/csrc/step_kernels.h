@@ -215,40 +215,59 @@ struct CPair {
 __host__ __device__ constexpr CPair cpair(double c) { return {(float)c, (float)(c - (double)(float)c)}; }
 __device__ __forceinline__ float mulc(float x, CPair c) { return __builtin_fmaf(x, c.hi, x * c.lo); }
 
-// Moments of the deviations f - w: rho = 1 + sum dev, j = sum c dev.
+// BGK on deviations: g - w = (f - w) + omega ((f_eq - w) - (f - w)),
+// f_eq - w = w (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).  Populations are taken
+// in opposite pairs (c_e = -c_d): the pair sum feeds rho, the pair difference
+// feeds j, and the two equilibria share S = drho + rho (4.5 cu^2 - 1.5 u^2) and
+// differ only in the sign of A = 3 rho cu.
 template <int Q>
-__device__ __forceinline__ void moments_dev(const float* fd, float& drho, float* u) {
+__device__ __forceinline__ void collide_dev(float* fd, CPair om) {
     using L = Lat<Q>;
-    float r = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
+    float r = fd[L::rest], j0 = 0.f, j1 = 0.f, j2 = 0.f;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-        r += fd[d];
-        if (L::c(d, 0)) j0 += L::c(d, 0) > 0 ? fd[d] : -fd[d];
-        if (L::c(d, 1)) j1 += L::c(d, 1) > 0 ? fd[d] : -fd[d];
-        if (L::c(d, 2)) j2 += L::c(d, 2) > 0 ? fd[d] : -fd[d];
+        if (d == L::rest || d > L::opp(d)) continue;
+        const int e = L::opp(d);
+        const float sum = fd[d] + fd[e], dif = fd[d] - fd[e];
+        r += sum;
+        if (L::c(d, 0)) j0 = L::c(d, 0) > 0 ? j0 + dif : j0 - dif;
+        if (L::c(d, 1)) j1 = L::c(d, 1) > 0 ? j1 + dif : j1 - dif;
+        if (L::c(d, 2)) j2 = L::c(d, 2) > 0 ? j2 + dif : j2 - dif;
     }
     const float inv = 1.f / (1.f + r);
-    drho = r;
-    u[0] = j0 * inv; u[1] = j1 * inv; u[2] = j2 * inv;
-}
-
-// BGK on deviations: g - w = (f - w) + omega ((f_eq - w) - (f - w)),
-// f_eq - w = w (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).
-template <int Q>
-__device__ __forceinline__ void collide_dev(float* fd, CPair omega) {
-    using L = Lat<Q>;
-    float drho, u[3];
-    moments_dev<Q>(fd, drho, u);
-    const float rho = 1.f + drho;
-    const float usq15 = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const float u[3] = {j0 * inv, j1 * inv, j2 * inv};
+    const float rho = 1.f + r;
+    const float q[3] = {u[0] * u[0], u[1] * u[1], u[2] * u[2]};
+    const float usq15 = 1.5f * (q[0] + q[1] + q[2]);
+    const float r3 = 3.f * rho;
+    const float a[3] = {r3 * u[0], r3 * u[1], r3 * u[2]};
+    {
+        const int d = L::rest;
+        const float feqd = mulc(__builtin_fmaf(-rho, usq15, r), cpair(L::w(d)));
+        const float x = feqd - fd[d];
+        fd[d] = __builtin_fmaf(x, om.hi, __builtin_fmaf(x, om.lo, fd[d]));
+    }
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-        float cu = 0.f;
-        if (L::c(d, 0)) cu += L::c(d, 0) > 0 ? u[0] : -u[0];
-        if (L::c(d, 1)) cu += L::c(d, 1) > 0 ? u[1] : -u[1];
-        if (L::c(d, 2)) cu += L::c(d, 2) > 0 ? u[2] : -u[2];
-        const float feqd = mulc(drho + rho * (cu * (3.f + 4.5f * cu) - usq15), cpair(L::w(d)));
-        fd[d] = fd[d] + mulc(feqd - fd[d], omega);
+        if (d == L::rest || d > L::opp(d)) continue;
+        const int e = L::opp(d);
+        float A = 0.f, cu = 0.f, cu2 = 0.f;
+        int nz = 0, k1 = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (!L::c(d, k)) continue;
+            A = nz == 0 ? (L::c(d, k) > 0 ? a[k] : -a[k]) : (L::c(d, k) > 0 ? A + a[k] : A - a[k]);
+            cu = nz == 0 ? (L::c(d, k) > 0 ? u[k] : -u[k]) : (L::c(d, k) > 0 ? cu + u[k] : cu - u[k]);
+            k1 = nz == 0 ? k : k1;
+            ++nz;
+        }
+        cu2 = nz == 1 ? q[k1] : cu * cu;
+        const float S = __builtin_fmaf(rho, __builtin_fmaf(4.5f, cu2, -usq15), r);
+        const CPair w = cpair(L::w(d));
+        const float xd = mulc(S + A, w) - fd[d];
+        const float xe = mulc(S - A, w) - fd[e];
+        fd[d] = __builtin_fmaf(xd, om.hi, __builtin_fmaf(xd, om.lo, fd[d]));
+        fd[e] = __builtin_fmaf(xe, om.hi, __builtin_fmaf(xe, om.lo, fd[e]));
     }
 }
 
@@ -351,9 +370,62 @@ struct ZouHe<Q, float> {
     }
 };
 
-template <class P>
-__device__ __forceinline__ typename P::calc_t load_pop(const typename P::store_t* g, int64_t i) {
-    return (typename P::calc_t)g[i];
+// Element i of population array d (g + d*N).  For 2- and 4-byte storage the
+// access is a buffer load/store: the array's base goes into a resource
+// descriptor built with scalar instructions (the array is uniform), the lane
+// supplies a 32-bit byte offset (node counts stay below 2^30, so offsets fit),
+// and the descriptor's size (N elements) makes the hardware drop any access
+// outside the array.  No per-lane 64-bit address arithmetic is left.  FP64
+// arrays may exceed 4 GiB and use plain global addressing.
+template <class S>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_rsrc(const S* g, int d, int64_t N) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(g + (int64_t)d * N), (short)0,
+                                             (int)(uint32_t)((uint64_t)N * sizeof(S)), 0x00020000);
+}
+// BUF = false: plain global addressing, for the record path whose scattered,
+// conditional accesses would each need their own scalar descriptor set-up.
+template <class P, bool BUF = true>
+__device__ __forceinline__ typename P::calc_t ldp(const typename P::store_t* g, int d, int64_t N, int64_t i) {
+    using S = typename P::store_t;
+    if constexpr (BUF && sizeof(S) == 4) {
+        const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(pop_rsrc(g, d, N), (int)((uint32_t)i * 4u), 0, 0);
+        return (typename P::calc_t)__builtin_bit_cast(float, v);
+    } else if constexpr (BUF && sizeof(S) == 2) {
+        const unsigned short v = __builtin_amdgcn_raw_buffer_load_b16(pop_rsrc(g, d, N), (int)((uint32_t)i * 2u), 0, 0);
+        return (typename P::calc_t)__builtin_bit_cast(_Float16, v);
+    } else {
+        return (typename P::calc_t)g[(int64_t)d * N + i];
+    }
+}
+template <class S, bool BUF = true>
+__device__ __forceinline__ void stp(S* g, int d, int64_t N, int64_t i, S v) {
+    if constexpr (BUF && sizeof(S) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), pop_rsrc(g, d, N), (int)((uint32_t)i * 4u),
+                                              0, 0);
+    } else if constexpr (BUF && sizeof(S) == 2) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), pop_rsrc(g, d, N),
+                                              (int)((uint32_t)i * 2u), 0, 0);
+    } else {
+        g[(int64_t)d * N + i] = v;
+    }
+}
+
+// Conversion of a computed population to the storage type.  FP16 storage
+// rounds the FP32 result (the collision's last operation is an FMA rounded to
+// FP32) to FP16.  The compiler would otherwise fuse that FMA with the
+// conversion into one mixed-precision FMA that rounds once, straight to FP16,
+// in some kernels and not in others, and kernels that must agree bit for bit
+// would differ in the last place: the empty register constraint keeps the FP32
+// value.
+template <class S, class T>
+__device__ __forceinline__ S to_store(T v) {
+    if constexpr (sizeof(S) == 2) {
+        float x = (float)v;
+        asm volatile("" : "+v"(x));
+        return (S)x;
+    } else {
+        return (S)v;
+    }
 }
 
 // Streaming schemes.
@@ -397,9 +469,12 @@ __device__ __forceinline__ void write_pre(const StepArgs& a, int64_t n, const ty
 
 // ------------------------------------------------------------------- kernel
 
+template <int Q, class P>
+__device__ __forceinline__ void apply_bc(const StepArgs& a, int64_t r, int type, typename P::calc_t* f);
+
 // Bounce-back, stale populations and Zou-He of boundary record r on the
 // gathered populations f (miss: links whose source node does not exist).
-template <int Q, class P, int SCH>
+template <int Q, class P, int SCH, bool BUF = true>
 __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64_t n, uint32_t miss,
                                              typename P::calc_t* f) {
     using L = Lat<Q>;
@@ -408,22 +483,30 @@ __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64
     const int64_t N = a.n;
     const uint32_t bb = a.bbb[r];
     if (SCH == SCH_AA_LOCAL) miss = a.bmiss[r];
-    if (miss) {
-        const T* stale = (const T*)a.bstale;
+    // bounce-back: this node's own post-collision g[opp d] (SCH_AA_LOCAL:
+    // A[d][x] already holds it, kept there by the gather step)
+    if (SCH != SCH_AA_LOCAL && (miss & bb)) {
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
-            if (!((miss >> d) & 1u)) continue;
-            if ((bb >> d) & 1u) {
-                // bounce-back: this node's own post-collision g[opp d]
-                if (SCH == SCH_TWO) f[d] = load_pop<P>(gin, (int64_t)L::opp(d) * N + n);
-                if (SCH == SCH_AA_GATHER) f[d] = load_pop<P>(gin, (int64_t)d * N + n);
-                // SCH_AA_LOCAL: A[d][x] already holds it (kept by the gather step)
-            } else {
-                f[d] = stale[(int64_t)d * a.nrec + r];
-            }
+            if (!((miss & bb) >> d & 1u)) continue;
+            if (SCH == SCH_TWO) f[d] = ldp<P, BUF>(gin, L::opp(d), a.ns, n);
+            if (SCH == SCH_AA_GATHER) f[d] = ldp<P, BUF>(gin, d, a.ns, n);
         }
     }
-    const int type = a.btype[r];
+    // unlinked populations the reference never rewrites keep their value
+    if (miss & ~bb) {
+        const typename P::store_t* stale = (const typename P::store_t*)a.bstale;
+#pragma unroll
+        for (int d = 0; d < Q; ++d)
+            if ((miss & ~bb) >> d & 1u) f[d] = (T)stale[(int64_t)d * a.nrec + r];
+    }
+    apply_bc<Q, P>(a, r, a.btype[r], f);
+}
+
+// Zou-He of boundary record r when its type is a boundary condition.
+template <int Q, class P>
+__device__ __forceinline__ void apply_bc(const StepArgs& a, int64_t r, int type, typename P::calc_t* f) {
+    using T = typename P::calc_t;
     if (type >= 3) {
         const FaceTab& F = a.faces[type - 3];
         double uv[3] = {0.0, 0.0, 0.0};
@@ -439,7 +522,7 @@ __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64
 // Pre-state output (modes PRE / SNAP), collision and store of node n.
 // dst(d) (AA gather only): destination node of population d, -1 if none;
 // evaluated at the store so no per-direction index stays live in registers.
-template <int Q, class P, int MODE, int SCH, class Dst>
+template <int Q, class P, int MODE, int SCH, bool BUF = true, class Dst>
 __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::calc_t* f, Dst dst) {
     using L = Lat<Q>;
     using S = typename P::store_t;
@@ -459,13 +542,19 @@ __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::c
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         if (SCH == SCH_TWO) {
-            gout[(int64_t)d * N + n] = (S)g[d];
+            stp<S, BUF>(gout, d, a.ns, n, to_store<S>(g[d]));
         } else if (SCH == SCH_AA_LOCAL) {
-            gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
+            stp<S, BUF>(gout, L::opp(d), a.ns, n, to_store<S>(g[d]));
         } else {
             const int32_t t = dst(d);
-            if (t >= 0) gout[(int64_t)d * N + t] = (S)g[d];
-            else gout[(int64_t)L::opp(d) * N + n] = (S)g[d];
+            if constexpr (BUF) {
+                if (t >= 0) stp<S, BUF>(gout, d, a.ns, t, to_store<S>(g[d]));
+                else stp<S, BUF>(gout, L::opp(d), a.ns, n, to_store<S>(g[d]));
+            } else {
+                // one store to a per-lane selected slot
+                S* q = t >= 0 ? gout + (int64_t)d * a.ns + t : gout + (int64_t)L::opp(d) * a.ns + n;
+                *q = to_store<S>(g[d]);
+            }
         }
     }
 }
@@ -514,11 +603,11 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
             if (SCH == SCH_AA_LOCAL || d == L::rest) {
-                f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+                f[d] = ldp<P>(gin, d, a.ns, n);
                 dst[d] = (int32_t)n;
             } else {
                 const int64_t src = (int64_t)b_src[d] + lane;
-                f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
+                f[d] = ldp<P>(gin, (SCH == SCH_AA_GATHER ? L::opp(d) : d), a.ns, src);
                 if (SCH == SCH_AA_GATHER) dst[d] = b_dst[d] + lane;
             }
         }
@@ -531,10 +620,10 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
             if (SCH == SCH_AA_LOCAL || d == L::rest) {
-                f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+                f[d] = ldp<P>(gin, d, a.ns, n);
             } else {
                 const int32_t src = a.map[p - a.off[d]];
-                if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
+                if (src >= 0) f[d] = ldp<P>(gin, (SCH == SCH_AA_GATHER ? L::opp(d) : d), a.ns, src);
                 else { f[d] = T(0); miss |= 1u << d; }
                 if (SCH == SCH_AA_GATHER) dst[d] = a.map[p + a.off[d]];
             }
@@ -564,29 +653,41 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 // or of segment blocks (4 segments per 256-thread block).
 enum { PART_FUSED = 0, PART_RECORDS = 1, PART_SEGMENTS = 2 };
 
+// One boundary record per lane.  Every population has one source, chosen per
+// lane without branches: the neighbour x - c_d when it exists, else this
+// node's own slot (bounce-back) or the stale value (unlinked populations the
+// reference never rewrites); then one load.  Zou-He runs on boundary-condition
+// records only.
 template <int Q, class P, int MODE, int SCH>
 __device__ __forceinline__ void record_node(const StepArgs& a, int64_t r) {
     using L = Lat<Q>;
     using S = typename P::store_t;
     using T = typename P::calc_t;
     const S* gin = (const S*)a.gin;
-    const int64_t N = a.n, R = a.nrec;
+    const S* stale = (const S*)a.bstale;
+    const int64_t NS = a.ns, R = a.nrec;
     const int64_t n = a.rec_node[r];
+    const uint32_t bb = a.bbb[r];
+    const uint32_t lmiss = SCH == SCH_AA_LOCAL ? a.bmiss[r] : 0u;
     T f[Q];
-    uint32_t miss = 0;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
+        const S* st = stale + (int64_t)d * R + r;
+        const S* p;
         if (SCH == SCH_AA_LOCAL || d == L::rest) {
-            f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+            // AA local: a bounced-back population already sits in A[d][x]
+            p = ((lmiss & ~bb) >> d & 1u) ? st : gin + (int64_t)d * NS + n;
         } else {
-            const int32_t src = a.rsrc[(int64_t)d * R + r];
-            if (src >= 0) f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + src);
-            else { f[d] = T(0); miss |= 1u << d; }
+            const int32_t sd = a.rsrc[(int64_t)d * R + r];
+            const S* nb = gin + (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * NS + sd;
+            const S* own = gin + (int64_t)(SCH == SCH_TWO ? L::opp(d) : d) * NS + n;
+            p = sd >= 0 ? nb : ((bb >> d & 1u) ? own : st);
         }
+        f[d] = (T)*p;
     }
-    apply_record<Q, P, SCH>(a, r, n, miss, f);
+    apply_bc<Q, P>(a, r, a.btype[r], f);
     // AA gather: the node at x + c_d is the source of direction opp d
-    emit<Q, P, MODE, SCH>(a, n, f, [&](int d) {
+    emit<Q, P, MODE, SCH, false>(a, n, f, [&](int d) {
         return d == L::rest ? (int32_t)n : a.rsrc[(int64_t)L::opp(d) * R + r];
     });
 }
@@ -606,9 +707,9 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         if (SCH == SCH_AA_LOCAL || d == L::rest) {
-            f[d] = load_pop<P>(gin, (int64_t)d * N + n);
+            f[d] = ldp<P>(gin, d, a.ns, n);
         } else {
-            f[d] = load_pop<P>(gin, (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * N + row[2 + d] + lane);
+            f[d] = ldp<P>(gin, (SCH == SCH_AA_GATHER ? L::opp(d) : d), a.ns, row[2 + d] + lane);
         }
     }
     emit<Q, P, MODE, SCH>(a, n, f, [&](int d) { return d == L::rest ? (int32_t)n : row[2 + L::opp(d)] + lane; });
@@ -667,17 +768,19 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a) {
         collide_dev<Q>((float*)g, cpair(a.omega));
     }
 #pragma unroll
-    for (int d = 0; d < Q; ++d) gout[(int64_t)(SWAP ? L::opp(d) : d) * a.n + n] = (S)g[d];
+    for (int d = 0; d < Q; ++d) gout[(int64_t)(SWAP ? L::opp(d) : d) * a.ns + n] = to_store<S>(g[d]);
 }
 
 // Every pointer a launch dereferences must be set; a missing one is an error
 // returned to the caller, never a launch.
 inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
-    if (!a.gin || !a.faces || !a.wvals || a.wk < 0 || a.wk >= LBM_WCHUNK) return false;
+    if (!a.gin || !a.faces || !a.wvals || a.wk < 0 || a.wk >= LBM_WCHUNK || a.ns < a.n || (a.ns & 63)) return false;
     if (layout == LBM_LAYOUT_GENERIC && (!a.pnode || !a.map || !a.bbits || !a.bpre)) return false;
     if (layout != LBM_LAYOUT_GENERIC && ((a.nseg > 0 && !a.seg) || (a.nrec > 0 && (!a.rsrc || !a.rec_node))))
         return false;
     if (layout == LBM_LAYOUT_FUSED && (!a.wlist || a.nwaves != a.nseg + (a.nrec + 63) / 64)) return false;
+    // the fused layout runs the AA local step on the generic kernel
+    if (layout == LBM_LAYOUT_FUSED && sch == SCH_AA_LOCAL && (!a.bbits || !a.bpre)) return false;
     if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale || !a.bmiss)) return false;
     if (mode != LBM_MODE_PRE && !a.gout) return false;
     if (mode != LBM_MODE_STEP && (!a.fpre || !a.rho || !a.u)) return false;
@@ -689,6 +792,13 @@ template <int Q, class P, int MODE, int SCH>
 void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
     if (layout == LBM_LAYOUT_GENERIC) {
         hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, b);
+    } else if (layout == LBM_LAYOUT_FUSED && SCH == SCH_AA_LOCAL) {
+        // The AA local step touches only each node's own slots: one dense
+        // launch in node order, fully coalesced, record lanes fix up in place.
+        StepArgs c = b;
+        c.wreg = nullptr;
+        c.nblocks = (int)((b.n + 255) / 256);
+        hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, c);
     } else if (layout == LBM_LAYOUT_FUSED) {
         hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(b.nwaves, 4)), dim3(256), 0,
                            s, b);
@@ -727,7 +837,7 @@ hipError_t launch_step_t(int mode, int sch, int layout, const StepArgs& a, hipSt
 
 template <int Q, class P>
 hipError_t launch_init_t(const InitArgs& a, bool swap, hipStream_t s) {
-    if (!a.gout) return hipErrorInvalidValue;
+    if (!a.gout || a.ns < a.n) return hipErrorInvalidValue;
     const int grid = lbm_grid_blocks(a.n, 256);
     InitArgs b = a;
     b.nblocks = (int)((a.n + 255) / 256);

@@ -141,8 +141,9 @@ def main():
              "voxelize_ms": case["vox_ms"], "classify_ms": case["cls_ms"], "scheme": args.scheme,
              "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3),
              "layout": args.layout, "boundary_records": r["stats"]["boundary_records"],
-             "segments": r["stats"]["segments"],
-             "segment_lane_use": (n - r["stats"]["boundary_records"]) / max(1, 64 * r["stats"]["segments"]),
+             "row_chunks" if args.layout == "fused" else "segments": r["stats"]["segments"],
+             "lane_use": ((n if args.layout == "fused" else n - r["stats"]["boundary_records"])
+                          / max(1, 64 * r["stats"]["segments"])),
              "node_order_fast_axis": r["stats"]["fast_axis"]}
     if args.layout == "generic":
         extra["regular_wave_fraction"] = r["stats"]["regular_waves"] / max(1, r["stats"]["waves"])

@@ -172,7 +172,7 @@ int lbm_domain_info(const lbm_domain* dom, int64_t* n_nodes, int64_t* n_boundary
 typedef struct {
     int64_t nodes, boundary_records, wall_nodes, bc_nodes;
     int64_t waves, regular_waves;  /* generic kernel: 64-node wavefronts; those on the row fast path */
-    int64_t segments;              /* record/segment kernel: row segments (0 for the generic kernel) */
+    int64_t segments;              /* row chunks (one-launch layout) or row segments (two-launch layout) */
     int64_t device_bytes;
     int32_t fast_axis;             /* node-order fastest axis */
     int32_t scheme;

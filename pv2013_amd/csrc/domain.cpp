@@ -111,8 +111,8 @@ struct lbm_domain {
     int32_t* rsrc = nullptr;   // record/segment layouts: [d][r] record sources
     int32_t* seg = nullptr;    // [s][LBM_SEG_W] row segments
     int64_t nseg = 0;
-    int32_t* wlist = nullptr;  // wavefront work list (records and segments in node order)
-    int64_t nwaves = 0;
+    int32_t* chunk = nullptr;  // fused layout: row chunks
+    int64_t nchunk = 0;
     std::vector<int32_t> wused;      // waveform slots referenced by a boundary condition
     int32_t* d_wused = nullptr;
     double* wvals = nullptr;         // [LBM_MAXWAVE][LBM_WCHUNK]
@@ -264,8 +264,8 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.rsrc = D->rsrc;
     a.seg = D->seg;
     a.nseg = D->nseg;
-    a.wlist = D->wlist;
-    a.nwaves = D->nwaves;
+    a.chunk = D->chunk;
+    a.nchunk = D->nchunk;
     a.bprof = D->bprof;
     a.bstale = D->bstale;
     a.faces = D->d_faces;
@@ -521,14 +521,14 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             D->nreg_waves++;
         }
     }
-    // Record sources and row segments (record/segment kernel).  A segment
+    // Two-launch layout: record sources and row segments.  A segment
     // grows node by node while the next node is interior, follows the last
     // one along the fast axis, and its source in every direction follows the
     // last one's source; segments also end at multiples of 64 node ids so a
     // wavefront's stores cover whole cache lines.  Every run this accepts is
     // contiguous by construction, and the bounds are checked again below.
-    std::vector<int32_t> rsrc, segs, wlist;
-    if (D->layout != LBM_LAYOUT_GENERIC) {
+    std::vector<int32_t> rsrc, segs, chunks;
+    if (D->layout == LBM_LAYOUT_SPLIT) {
         rsrc.assign((size_t)q * std::max<int64_t>(D->nrec, 1), -1);
         for (int64_t r = 0; r < D->nrec; ++r) {
             const int32_t m = rec_node[r];
@@ -565,16 +565,76 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             }
         }
         if (D->nseg == 0) segs.assign(LBM_SEG_W, 0);
-        // merge segments and 64-record chunks by first node id
-        const int64_t nchunk = (D->nrec + 63) / 64;
-        wlist.reserve((size_t)(D->nseg + nchunk));
-        int64_t si = 0, ci = 0;
-        while (si < D->nseg || ci < nchunk) {
-            const bool take_seg = ci >= nchunk || (si < D->nseg && segs[si * LBM_SEG_W] < rec_node[ci * 64]);
-            if (take_seg) wlist.push_back((int32_t)si++);
-            else wlist.push_back((int32_t)(-(ci++) - 1));
+    }
+    // Fused layout: row chunks.  A chunk grows node by node while the next
+    // node follows the last one along the fast axis inside the same 64-node
+    // word and, for every direction, its source either is missing or continues
+    // the run of consecutive sources the chunk has so far (a direction whose
+    // sources stopped may not start again).  So per direction the lanes with a
+    // source form one interval [lo, hi) reading base + lane.
+    if (D->layout == LBM_LAYOUT_FUSED) {
+        const int CW = lbm_chunk_w(q);
+        std::vector<int32_t> cbase(q, 0), clo(q, -1), chi(q, -1);
+        int64_t m = 0;
+        while (m < D->n) {
+            const int64_t start = m;
+            int len = 0;
+            for (;;) {
+                const int64_t x = start + len;
+                if (x >= D->n || len == 64 || (len > 0 && ((x & 63) == 0 || pnode[x] != pnode[x - 1] + 1))) break;
+                bool ok = true;
+                for (int e = 0; ok && len > 0 && e < q; ++e) {
+                    if (e == rest) continue;
+                    const int32_t sx = map[pnode[x] - D->off[e]];
+                    if (sx >= 0 && clo[e] >= 0 && (chi[e] != len || sx != cbase[e] + len)) ok = false;
+                }
+                if (!ok) break;
+                for (int e = 0; e < q; ++e) {
+                    if (e == rest) continue;
+                    if (len == 0) clo[e] = chi[e] = -1;
+                    const int32_t sx = map[pnode[x] - D->off[e]];
+                    if (sx < 0) continue;
+                    if (clo[e] < 0) {
+                        clo[e] = len;
+                        chi[e] = len + 1;
+                        cbase[e] = sx - len;
+                    } else {
+                        chi[e] = len + 1;
+                    }
+                }
+                ++len;
+            }
+            const size_t c0 = chunks.size();
+            chunks.resize(c0 + CW, 0);
+            int32_t* h = &chunks[c0];
+            const uint64_t word = bits[start >> 6];
+            const int sh = (int)(start & 63);
+            h[0] = (int32_t)start;
+            h[1] = len;
+            h[2] = pre[start >> 6] + (sh ? __builtin_popcountll(word & ((1ull << sh) - 1ull)) : 0);
+            const uint64_t rm = (word >> sh) & (len == 64 ? ~0ull : ((1ull << len) - 1ull));
+            h[4] = (int32_t)(uint32_t)rm;
+            h[5] = (int32_t)(uint32_t)(rm >> 32);
+            bool ok = len >= 1 && sh + len <= 64;
+            for (int e = 0; ok && e < q; ++e) {
+                if (e == rest) continue;
+                if (clo[e] >= 0) {
+                    h[6 + 2 * e] = cbase[e];
+                    h[7 + 2 * e] = clo[e] | (chi[e] << 8);
+                    ok = (int64_t)cbase[e] + clo[e] >= 0 && (int64_t)cbase[e] + chi[e] <= D->n;
+                }
+                // a lane without a record has every link
+                const int lo_e = clo[e] < 0 ? 0 : clo[e], hi_e = clo[e] < 0 ? 0 : chi[e];
+                for (int l = 0; ok && l < len; ++l)
+                    if (!((rm >> l) & 1ull) && (l < lo_e || l >= hi_e)) ok = false;
+            }
+            if (!ok) {
+                delete D;
+                return lbm_ctx_fail(ctx, LBM_EIO, "internal error: bad row chunk");
+            }
+            m = start + len;
         }
-        D->nwaves = (int64_t)wlist.size();
+        D->nchunk = (int64_t)(chunks.size() / CW);
     }
     // device allocation
     int rc = LBM_OK;
@@ -603,11 +663,11 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->bmiss, (size_t)D->nrec));
     DCHECK(D->alloc(&D->rec_node, (size_t)D->nrec));
     DCHECK(D->alloc(&D->bprof, (size_t)D->nrec));
-    if (D->layout != LBM_LAYOUT_GENERIC) {
+    if (D->layout == LBM_LAYOUT_SPLIT) {
         DCHECK(D->alloc(&D->rsrc, rsrc.size()));
         DCHECK(D->alloc(&D->seg, segs.size()));
-        DCHECK(D->alloc(&D->wlist, std::max<size_t>(wlist.size(), 1)));
     }
+    if (D->layout == LBM_LAYOUT_FUSED) DCHECK(D->alloc(&D->chunk, chunks.size()));
     uint8_t* stale_raw = nullptr;
     DCHECK(D->alloc(&stale_raw, (size_t)q * std::max<int64_t>(D->nrec, 1) * stale_sb));
     D->bstale = stale_raw;
@@ -648,11 +708,13 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if ((rc = upload(D, D->bmiss, bmiss.data(), sizeof(uint32_t) * bmiss.size()))) return fail(rc);
     if ((rc = upload(D, D->rec_node, rec_node.data(), sizeof(int32_t) * rec_node.size()))) return fail(rc);
     if ((rc = upload(D, D->bprof, bprof.data(), sizeof(double) * bprof.size()))) return fail(rc);
-    if (D->layout != LBM_LAYOUT_GENERIC) {
+    if (D->layout == LBM_LAYOUT_SPLIT) {
         if ((rc = upload(D, D->rsrc, rsrc.data(), sizeof(int32_t) * rsrc.size()))) return fail(rc);
         if ((rc = upload(D, D->seg, segs.data(), sizeof(int32_t) * segs.size()))) return fail(rc);
-        if ((rc = upload(D, D->wlist, wlist.data(), sizeof(int32_t) * wlist.size()))) return fail(rc);
     }
+    if (D->layout == LBM_LAYOUT_FUSED &&
+        (rc = upload(D, D->chunk, chunks.data(), sizeof(int32_t) * chunks.size())))
+        return fail(rc);
     if ((rc = upload(D, D->d_faces, D->faces.data(), sizeof(FaceTab) * D->faces.size()))) return fail(rc);
     if ((rc = upload(D, D->d_waves, D->waves.data(), sizeof(WaveTab) * D->waves.size()))) return fail(rc);
     if ((rc = upload(D, D->d_wused, D->wused.data(), sizeof(int32_t) * D->wused.size()))) return fail(rc);
@@ -701,7 +763,7 @@ extern "C" int lbm_domain_get_stats(const lbm_domain* D, lbm_domain_stats* o) {
     o->bc_nodes = (int64_t)D->bc_nodes.size();
     o->waves = (D->n + 63) / 64;
     o->regular_waves = D->nreg_waves;
-    o->segments = D->nseg;
+    o->segments = D->layout == LBM_LAYOUT_FUSED ? D->nchunk : D->nseg;
     o->layout = D->layout;
     o->device_bytes = D->device_bytes;
     o->fast_axis = D->perm[0];

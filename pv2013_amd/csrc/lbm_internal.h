@@ -50,8 +50,8 @@ struct StepArgs {
     const int32_t* rsrc;    // [d][r] source node of record r in direction d (x - c_d), -1: none
     const int32_t* seg;     // [s][LBM_SEG_W] row segments: start, length, source base per direction
     int64_t nseg;
-    const int32_t* wlist;   // wavefront work list in node order: s >= 0 segment s, -(k+1) records 64k..64k+63
-    int64_t nwaves;
+    const int32_t* chunk;   // [c][lbm_chunk_w(q)] row chunks (fused layout)
+    int64_t nchunk;
     int32_t rb8, sb8;       // record / segment blocks per XCD (two-launch layout, set by the launcher)
     const double* bprof;    // per record: velocity profile factor
     const void* bstale;     // [e][r] stale populations (double, or float deviations)
@@ -99,6 +99,15 @@ enum { LBM_LAYOUT_GENERIC = 0, LBM_LAYOUT_FUSED = 1, LBM_LAYOUT_SPLIT = 2 };
 // ints; base[d] is the node id at x_start - c_d, so lane l of the segment reads
 // direction d from node base[d] + l.
 constexpr int LBM_SEG_W = 32;
+
+// Row chunk table entry (fused layout): every node belongs to one chunk, up to
+// 64 consecutive node ids along one row inside one 64-node word.
+//   [0] start node, [1] length, [2] records before start, [3] unused,
+//   [4], [5] record mask (bit l: node start + l has a boundary record),
+//   [6 + 2d] base_d, [7 + 2d] lo_d | hi_d << 8: lanes l in [lo_d, hi_d) read
+//   direction d from node base_d + l; the other lanes' source x - c_d is
+//   missing (bounce-back or stale value).
+constexpr int lbm_chunk_w(int q) { return q == 19 ? 48 : 64; }
 
 // Waveform values are tabulated once per chunk of LBM_WCHUNK consecutive
 // iterations: wvals[slot][k] = W_slot(base + k), base = it0 (it_abs) or *iter.

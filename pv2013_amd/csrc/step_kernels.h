@@ -644,13 +644,11 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 // in each direction are again consecutive ids, so one wavefront reads each
 // direction as one contiguous run from a base in the segment table.
 //
-// One launch (PART_FUSED): each 64-lane wavefront takes one entry of a work
-// list ordered by first node id, either a segment or 64 consecutive records,
-// so records and the segments around them (which share cache lines of every
-// population array) run at the same time on the same XCD.  Blocks map to XCDs
-// in contiguous ranges (logical_block).  Two launches (PART_RECORDS, then
-// PART_SEGMENTS): per XCD (block id mod 8) a contiguous share of record blocks
-// or of segment blocks (4 segments per 256-thread block).
+// One launch (PART_FUSED): each 64-lane wavefront takes one row chunk, records
+// and interior nodes alike, in node order; blocks map to XCDs in contiguous
+// ranges (logical_block).  Two launches (PART_RECORDS, then PART_SEGMENTS):
+// per XCD (block id mod 8) a contiguous share of record blocks or of segment
+// blocks (4 segments per 256-thread block).
 enum { PART_FUSED = 0, PART_RECORDS = 1, PART_SEGMENTS = 2 };
 
 // One boundary record per lane.  Every population has one source, chosen per
@@ -715,21 +713,71 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
     emit<Q, P, MODE, SCH>(a, n, f, [&](int d) { return d == L::rest ? (int32_t)n : row[2 + L::opp(d)] + lane; });
 }
 
+// One node of a row chunk (lane l = node start + l).  A chunk without records
+// (the common case, a wave-uniform test) reads every direction as one
+// contiguous run base_d + l, exactly like a segment.  A chunk with records
+// picks, per lane and direction and without branches, the neighbour base_d + l
+// on the lanes of [lo_d, hi_d), else the node's own slot (bounce-back) or its
+// stale value, and issues one load; record lanes find their record index from
+// the chunk's record mask.
+template <int Q, class P, int MODE, int SCH>
+__device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lane) {
+    using L = Lat<Q>;
+    using S = typename P::store_t;
+    using T = typename P::calc_t;
+    const int32_t* h = a.chunk + c * lbm_chunk_w(Q);  // wave-uniform
+    const int32_t start = h[0], len = h[1];
+    if (lane >= len) return;
+    const S* gin = (const S*)a.gin;
+    const int64_t NS = a.ns;
+    const int64_t n = (int64_t)start + lane;
+    const uint32_t mlo = (uint32_t)h[4], mhi = (uint32_t)h[5];
+    T f[Q];
+    if ((mlo | mhi) == 0u) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            if (SCH == SCH_AA_LOCAL || d == L::rest) f[d] = ldp<P>(gin, d, NS, n);
+            else f[d] = ldp<P>(gin, SCH == SCH_AA_GATHER ? L::opp(d) : d, NS, (int64_t)h[6 + 2 * d] + lane);
+        }
+        emit<Q, P, MODE, SCH>(a, n, f, [&](int d) { return d == L::rest ? (int32_t)n : h[6 + 2 * L::opp(d)] + lane; });
+        return;
+    }
+    const bool isrec = lane < 32 ? (mlo >> lane) & 1u : (mhi >> (lane - 32)) & 1u;
+    const int64_t r = (int64_t)h[2] + __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
+    const uint32_t bb = isrec ? a.bbb[r] : 0u;
+    const uint32_t lmiss = (SCH == SCH_AA_LOCAL && isrec) ? a.bmiss[r] : 0u;
+    const S* stale = (const S*)a.bstale;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        const S* st = stale + (int64_t)d * a.nrec + r;
+        const S* p;
+        if (SCH == SCH_AA_LOCAL || d == L::rest) {
+            p = ((lmiss & ~bb) >> d & 1u) ? st : gin + (int64_t)d * NS + n;
+        } else {
+            const int32_t lh = h[7 + 2 * d];
+            const bool pres = lane >= (lh & 0xff) && lane < ((lh >> 8) & 0xff);
+            const S* nb = gin + (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * NS + ((int64_t)h[6 + 2 * d] + lane);
+            const S* own = gin + (int64_t)(SCH == SCH_TWO ? L::opp(d) : d) * NS + n;
+            p = pres ? nb : ((bb >> d & 1u) ? own : st);
+        }
+        f[d] = (T)*p;
+    }
+    if (isrec) apply_bc<Q, P>(a, r, a.btype[r], f);
+    // AA gather: the node at x + c_d is the source of direction opp d
+    emit<Q, P, MODE, SCH, false>(a, n, f, [&](int d) {
+        if (d == L::rest) return (int32_t)n;
+        const int32_t lh = h[7 + 2 * L::opp(d)];
+        return (lane >= (lh & 0xff) && lane < ((lh >> 8) & 0xff)) ? h[6 + 2 * L::opp(d)] + lane : -1;
+    });
+}
+
 template <int Q, class P, int MODE, int SCH, int PART>
 __global__ __launch_bounds__(256) void k_step_rs(StepArgs a) {
     if constexpr (PART == PART_FUSED) {
         const int64_t lb = logical_block();
         if (lb >= a.nblocks) return;
-        const int64_t w = __builtin_amdgcn_readfirstlane((int)(lb * 4 + (threadIdx.x >> 6)));
-        if (w >= a.nwaves) return;
-        const int32_t e = a.wlist[w];  // wave-uniform
-        const int lane = (int)(threadIdx.x & 63);
-        if (e >= 0) {
-            segment_node<Q, P, MODE, SCH>(a, e, lane);
-        } else {
-            const int64_t r = (int64_t)(-(e + 1)) * 64 + lane;
-            if (r < a.nrec) record_node<Q, P, MODE, SCH>(a, r);
-        }
+        const int64_t c = __builtin_amdgcn_readfirstlane((int)(lb * 4 + (threadIdx.x >> 6)));
+        if (c < a.nchunk) chunk_node<Q, P, MODE, SCH>(a, c, (int)(threadIdx.x & 63));
         return;
     }
     const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
@@ -776,9 +824,10 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a) {
 inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
     if (!a.gin || !a.faces || !a.wvals || a.wk < 0 || a.wk >= LBM_WCHUNK || a.ns < a.n || (a.ns & 63)) return false;
     if (layout == LBM_LAYOUT_GENERIC && (!a.pnode || !a.map || !a.bbits || !a.bpre)) return false;
-    if (layout != LBM_LAYOUT_GENERIC && ((a.nseg > 0 && !a.seg) || (a.nrec > 0 && (!a.rsrc || !a.rec_node))))
+    if (layout == LBM_LAYOUT_SPLIT && ((a.nseg > 0 && !a.seg) || (a.nrec > 0 && (!a.rsrc || !a.rec_node))))
         return false;
-    if (layout == LBM_LAYOUT_FUSED && (!a.wlist || a.nwaves != a.nseg + (a.nrec + 63) / 64)) return false;
+    if (layout == LBM_LAYOUT_SPLIT && a.nseg + a.nrec <= 0) return false;
+    if (layout == LBM_LAYOUT_FUSED && (!a.chunk || a.nchunk <= 0)) return false;
     // the fused layout runs the AA local step on the generic kernel
     if (layout == LBM_LAYOUT_FUSED && sch == SCH_AA_LOCAL && (!a.bbits || !a.bpre)) return false;
     if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale || !a.bmiss)) return false;
@@ -792,16 +841,18 @@ template <int Q, class P, int MODE, int SCH>
 void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
     if (layout == LBM_LAYOUT_GENERIC) {
         hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, b);
-    } else if (layout == LBM_LAYOUT_FUSED && SCH == SCH_AA_LOCAL) {
-        // The AA local step touches only each node's own slots: one dense
-        // launch in node order, fully coalesced, record lanes fix up in place.
-        StepArgs c = b;
-        c.wreg = nullptr;
-        c.nblocks = (int)((b.n + 255) / 256);
-        hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, c);
     } else if (layout == LBM_LAYOUT_FUSED) {
-        hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(b.nwaves, 4)), dim3(256), 0,
-                           s, b);
+        if constexpr (SCH == SCH_AA_LOCAL) {
+            // The AA local step touches only each node's own slots: one dense
+            // launch in node order, fully coalesced, record lanes fix up in place.
+            StepArgs c = b;
+            c.wreg = nullptr;
+            c.nblocks = (int)((b.n + 255) / 256);
+            hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, c);
+        } else {
+            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(b.nchunk, 4)), dim3(256),
+                               0, s, b);
+        }
     } else {
         // records first, then segments: the two launches write disjoint nodes
         if (b.rb8 > 0)
@@ -825,10 +876,9 @@ template <int Q, class P>
 hipError_t launch_step_t(int mode, int sch, int layout, const StepArgs& a, hipStream_t s) {
     if (!step_args_ok(mode, sch, layout, a)) return hipErrorInvalidValue;
     StepArgs b = a;
-    b.nblocks = (int)(layout == LBM_LAYOUT_FUSED ? (a.nwaves + 3) / 4 : (a.n + 255) / 256);
+    b.nblocks = (int)(layout == LBM_LAYOUT_FUSED ? (a.nchunk + 3) / 4 : (a.n + 255) / 256);
     b.rb8 = (int)(((a.nrec + 255) / 256 + 7) / 8);
     b.sb8 = (int)(((a.nseg + 3) / 4 + 7) / 8);
-    if (layout != LBM_LAYOUT_GENERIC && b.rb8 + b.sb8 == 0) return hipErrorInvalidValue;
     if (mode == LBM_MODE_STEP) launch_sch<Q, P, LBM_MODE_STEP>(sch, layout, b, s);
     else if (mode == LBM_MODE_SNAP) launch_sch<Q, P, LBM_MODE_SNAP>(sch, layout, b, s);
     else launch_layout<Q, P, LBM_MODE_PRE, SCH_TWO>(layout, b, s);

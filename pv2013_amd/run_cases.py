@@ -79,7 +79,7 @@ def run_c4(ctx, quick):
         per_case = [dom.macros_box(inf["origin"], inf["dims"])[1] for inf in info]
         res[name] = {"mlups": stats["nodes"] * (steps - 64) / (ms * 1e-3) / 1e6, "nodes": stats["nodes"],
                      "u": per_case,
-                     "segment_lane_use": (stats["nodes"] - stats["boundary_records"]) / max(1, 64 * stats["segments"])}
+                     "lane_use": stats["nodes"] / max(1, 64 * stats["segments"])}
         dom.close()
     errs = [cases.errors(u64, u32, c["flags"] > 0)["l2"]
             for u64, u32, c in zip(res["f64"]["u"], res["f32"]["u"], cs)]
@@ -94,7 +94,7 @@ def run_c4(ctx, quick):
             "roofline_frac_f32": res["f32"]["mlups"] * 152 / 1e3 / HBM,
             "velocity_l2_vs_f64": {"max": max(errs), "median": float(np.median(errs))},
             "packed_f64_equals_single_case": same,
-            "segment_lane_use": res["f32"]["segment_lane_use"]}
+            "lane_use": res["f32"]["lane_use"]}
 
 
 def run_c5(ctx, quick):

@@ -131,7 +131,7 @@ def test_fp64_waveform_within_1e12(P, oracle, ctx):
     assert np.linalg.norm(u1 - u2) <= 1e-12 * np.linalg.norm(u2)
 
 
-@pytest.mark.parametrize("storage,bound", [(1, 1e-5), (2, 5e-3)])
+@pytest.mark.parametrize("storage,bound", [(1, 2e-6), (2, 5e-3)])
 def test_low_precision_error_vs_fp64(P, oracle, ctx, storage, bound):
     dims = (32, 32, 128)
     cl, bcs = pipe_case(oracle, dims, 12.0, 0.05)
@@ -144,7 +144,7 @@ def test_low_precision_error_vs_fp64(P, oracle, ctx, storage, bound):
     _, ulp = lp.macros()
     err = np.linalg.norm(ulp - u64) / np.linalg.norm(u64)
     print("storage %d velocity L2 rel error %.3e" % (storage, err), flush=True)
-    assert err < bound
+    assert err < bound, "storage %d: velocity L2 error %.3e vs FP64 exceeds %.1e" % (storage, err, bound)
 
 
 def test_mass_conserved_closed_box_clean(P, oracle, ctx):

@@ -305,9 +305,11 @@ def vessel_tree(seed, dims=(768, 384, 384), r_root=60.0, r_leaf=15.0, levels=3, 
             yc = qy + (py - qy) * sm
             zc = qz + (pz - qz) * sm
             dt = _tube_dist(Y - yc, Z - zc, r, md, X)
-            dt = np.maximum(dt, (xs - r * 0.5) - X)
+            # flat tube ends sit off lattice planes (irrational fractions): a
+            # voxelizer ray running inside a cap's plane would be degenerate
+            dt = np.maximum(dt, (xs - r * 0.5 - 0.2718281828) - X)
             if lev < levels:  # a parent ends inside its children's start
-                dt = np.maximum(dt, X - (xs + seglen + 0.5 * r))
+                dt = np.maximum(dt, X - (xs + seglen + 0.5 * r + 0.3141592654))
             d = np.minimum(d, dt)
         return np.maximum(d, np.maximum(x0 - X, X - x1))
 

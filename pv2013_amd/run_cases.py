@@ -68,7 +68,8 @@ def run_c4(ctx, quick):
     res = {}
     for name, st in (("f32", P.STORE_F32), ("f64", P.STORE_F64)):
         dom, info = P.batch(ctx, [dict(dims=c["dims"], flags=c["flags"], tau=c["tau"],
-                                       bcs=[P.make_bc(**b) for b in c["bcs"]], q=19, storage=st)
+                                       bcs=[P.make_bc(**b) for b in c["bcs"]], q=19, storage=st,
+                                       scheme=P.SCHEME_AA)
                                   for c in cs])
         for c, inf in zip(cs, info):
             for slot, (kind, period, a, phi) in c["waves"].items():
@@ -85,7 +86,7 @@ def run_c4(ctx, quick):
             for u64, u32, c in zip(res["f64"]["u"], res["f32"]["u"], cs)]
     # the packed FP64 run equals each case run alone, bit for bit: check one
     c = cs[0]
-    single = cases.make_domain(ctx, c, storage=P.STORE_F64)
+    single = cases.make_domain(ctx, c, storage=P.STORE_F64, scheme=P.SCHEME_AA)
     single.step(steps)
     same = bool(np.array_equal(single.macros()[1], res["f64"]["u"][0]))
     single.close()
@@ -104,14 +105,14 @@ def run_c5(ctx, quick):
     t0 = time.perf_counter()
     c = cases.c5(ctx, seed=5, dims=dims, h=4.0 if quick else 3.0)
     gen = time.perf_counter() - t0
-    dom = cases.make_domain(ctx, c, q=27, storage=P.STORE_F16S)
+    dom = cases.make_domain(ctx, c, q=27, storage=P.STORE_F16S, scheme=P.SCHEME_AA)
     st = dom.stats()
     dom.step(cmp_steps)
     w16 = dom.wss()
     ms = dom.step_timed(steps - cmp_steps)
     mass, _ = dom.reduce()
     dom.close()
-    ref = cases.make_domain(ctx, c, q=27, storage=P.STORE_F64)
+    ref = cases.make_domain(ctx, c, q=27, storage=P.STORE_F64, scheme=P.SCHEME_AA)
     ref.step(cmp_steps)
     w64 = ref.wss()
     ms64 = ref.step_timed(200)

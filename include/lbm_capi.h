@@ -53,6 +53,13 @@ int lbm_ctx_create(int device, lbm_ctx** out);
 void lbm_ctx_destroy(lbm_ctx* ctx);
 /* Text of the last error on this context (never NULL). */
 const char* lbm_last_error(const lbm_ctx* ctx);
+
+/* Measured device bandwidth for the roofline next to the published HBM peak:
+ * a streaming kernel over a scratch buffer of `bytes` (a multiple of 16),
+ * in place (read and write the same buffer) or as a copy between two buffers,
+ * repeated `reps` times after one warm-up pass; *gbs = bytes read + written per
+ * second, in GB/s, from HIP events.  No reference counterpart. */
+int lbm_ctx_bandwidth(lbm_ctx* ctx, int64_t bytes, int in_place, int reps, double* gbs);
 int lbm_version(void);
 
 /* ----------------------------------------------------------------- geometry */

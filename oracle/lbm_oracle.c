@@ -51,6 +51,9 @@
 #include "lbm_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -701,6 +704,82 @@ static void step_once(orc_sim* s) {
 
 void orc_sim_step(orc_sim* s, int64_t nsteps) {
     for (int64_t i = 0; i < nsteps; ++i) step_once(s);
+}
+
+/* The same step with OpenMP over the nodes of each pass, for the timed CPU
+ * baseline.  Collision and the boundary pass touch one node each; each
+ * destination slot of the push has exactly one source.  The only read that
+ * the sequential pass orders is the f != 0 guard (it may see a value pushed
+ * earlier in the same pass): here the guard is taken on the pre-collision f,
+ * recorded during collision.  An FP64 population is never exactly 0.0 (f is
+ * about w rho), so both read the guard as true and give the same bits; the
+ * tests compare the two step functions. */
+static void step_once_threads(orc_sim* s, uint8_t* guard) {
+    const int q = s->q;
+    const int32_t* opp = opps(q);
+    const int64_t n = s->n;
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < n; ++m) {
+        double rho, u[3];
+        for (int d = 0; d < q; ++d) guard[m * q + d] = s->f[m * q + d] != 0.0;
+        orc_collide_node(q, s->omega, s->f + m * q, s->fnew + m * q, &rho, u);
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < n; ++m)
+        for (int d = 0; d < q; ++d) {
+            int64_t t = s->nb[m * q + d];
+            if (t >= 0 && guard[m * q + d]) s->f[t * q + d] = s->fnew[m * q + d];
+        }
+    double wv[MAXWAVE];
+    int used[MAXWAVE] = {0};
+    for (int b = 0; b < s->nbc; ++b)
+        if (s->bcs[b].waveform >= 0 && s->bcs[b].waveform < MAXWAVE) used[s->bcs[b].waveform] = 1;
+    for (int w = 0; w < MAXWAVE; ++w) wv[w] = used[w] ? wave_eval(s, w, s->iter, 1) : 1.0;
+    const int rd = rest_dir(q);
+    const int d0 = (s->quirks && q == 19) ? 1 : 0;
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < n; ++m) {
+        int t = s->type[m];
+        int bb = s->quirks ? (t == 2) : 1;
+        if (bb) {
+            for (int d = d0; d < q; ++d) {
+                if (d == rd) continue;
+                if (s->nb[m * q + d] < 0) s->f[m * q + opp[d]] = s->fnew[m * q + d];
+            }
+        }
+        if (t >= 3 && t - 3 < s->nbc) {
+            const orc_bc* bc = &s->bcs[t - 3];
+            double uv[3] = {0, 0, 0};
+            if (bc->kind == ORC_BC_VELOCITY) {
+                double w = bc->waveform >= 0 ? wv[bc->waveform] : 1.0;
+                double fac = s->prof[m] * w;
+                for (int a = 0; a < 3; ++a) uv[a] = bc->u[a] * fac;
+            }
+            orc_zouhe_node(q, bc, uv, s->f + m * q);
+        }
+    }
+    s->iter++;
+}
+
+int orc_sim_step_threads(orc_sim* s, int64_t nsteps, int nthreads) {
+    uint8_t* guard = (uint8_t*)malloc((size_t)(s->n ? s->n : 1) * s->q);
+    if (!guard) return -12;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    for (int64_t i = 0; i < nsteps; ++i) step_once_threads(s, guard);
+    free(guard);
+    return 0;
+}
+
+int orc_threads_available(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 void orc_sim_get_f_dense(const orc_sim* s, double* f) {

@@ -89,6 +89,10 @@ void orc_sim_destroy(orc_sim* s);
 int orc_sim_set_waveform(orc_sim* s, int slot, int kind, double period, int nk,
                          const double* a, const double* phi, double dt);
 void orc_sim_step(orc_sim* s, int64_t nsteps);
+/* The same steps with OpenMP threads (timed CPU baseline); identical results
+ * whenever no population is exactly 0.0.  Returns 0, or -12 without memory. */
+int orc_sim_step_threads(orc_sim* s, int64_t nsteps, int nthreads);
+int orc_threads_available(void);
 int64_t orc_sim_nodes(const orc_sim* s);
 int64_t orc_sim_iter(const orc_sim* s);
 /* dense arrays over the box, site = i + nx*(j + ny*k); solid sites get 0 */

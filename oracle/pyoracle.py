@@ -64,6 +64,9 @@ def lib():
         L.orc_sim_set_waveform.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
                                            P(np.float64), P(np.float64), C.c_double]
         L.orc_sim_step.argtypes = [C.c_void_p, C.c_int64]
+        L.orc_sim_step_threads.argtypes = [C.c_void_p, C.c_int64, C.c_int]
+        L.orc_sim_step_threads.restype = C.c_int
+        L.orc_threads_available.restype = C.c_int
         L.orc_sim_nodes.argtypes = [C.c_void_p]
         L.orc_sim_nodes.restype = C.c_int64
         L.orc_sim_iter.argtypes = [C.c_void_p]
@@ -206,6 +209,10 @@ class Sim:
     def step(self, n=1):
         lib().orc_sim_step(self.h, n)
 
+    def step_threads(self, n=1, threads=0):
+        """The same steps on OpenMP threads (0: the OpenMP default)."""
+        assert lib().orc_sim_step_threads(self.h, n, threads) == 0
+
     def f(self):
         out = np.zeros(self.q * self.nsites)
         lib().orc_sim_get_f_dense(self.h, out)
@@ -227,6 +234,10 @@ class Sim:
 
     def mass(self):
         return lib().orc_sim_mass(self.h)
+
+
+def threads_available():
+    return lib().orc_threads_available()
 
 
 def collide_node(q, omega, f):

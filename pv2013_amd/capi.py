@@ -126,6 +126,7 @@ def lib():
         L.lbm_batch_create.argtypes = [vp, vp, i32, i32, P(vp), vp]
         L.lbm_domain_get_stats.argtypes = [vp, P(DomainStats)]
         L.lbm_autotune.argtypes = [vp, P(DomainDesc), i32, C.c_char_p, P(TuneResult)]
+        L.lbm_ctx_bandwidth.argtypes = [vp, i64, i32, i32, P(dbl)]
         _lib = L
     return _lib
 
@@ -227,6 +228,14 @@ class Context:
         if rc:
             msg = lib().lbm_last_error(self.h).decode(errors="replace")
             raise LbmError("%s failed (%d): %s" % (what, rc, msg))
+
+    def bandwidth(self, nbytes, in_place=True, reps=20):
+        """Measured streaming bandwidth in GB/s (bytes read + written per
+        second) over a scratch buffer of nbytes (lbm_ctx_bandwidth)."""
+        g = C.c_double()
+        self.check(lib().lbm_ctx_bandwidth(self.h, int(nbytes) // 16 * 16, 1 if in_place else 0, reps, C.byref(g)),
+                   "lbm_ctx_bandwidth")
+        return g.value
 
     def voxelize(self, tri, grid):
         """Geometry::labelPoints on the GPU: uint8 0/1 flags, x fastest.

@@ -84,6 +84,33 @@ extern "C" void lbm_ctx_destroy(lbm_ctx* ctx) {
 
 extern "C" const char* lbm_last_error(const lbm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+extern "C" int lbm_ctx_bandwidth(lbm_ctx* ctx, int64_t bytes, int in_place, int reps, double* gbs) {
+    if (!ctx) return LBM_EINVAL;
+    if (bytes <= 0 || (bytes & 15) || reps <= 0 || !gbs)
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_ctx_bandwidth: bad argument");
+    CHECK(hipSetDevice(ctx->device));
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void**)&a, (size_t)bytes) != hipSuccess) return lbm_ctx_fail(ctx, LBM_ENOMEM, "probe buffer");
+    if (!in_place && hipMalloc((void**)&b, (size_t)bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return lbm_ctx_fail(ctx, LBM_ENOMEM, "probe buffer");
+    }
+    const int64_t n = bytes / 16;
+    hipError_t e = hipMemsetAsync(a, 0, (size_t)bytes, ctx->stream);
+    if (e == hipSuccess) e = launch_stream_probe(a, b, n, ctx->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stream_probe(a, b, n, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return lbm_ctx_fail(ctx, LBM_EIO, std::string("lbm_ctx_bandwidth: ") + hipGetErrorString(e));
+    *gbs = ms > 0.f ? 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9 : 0.0;
+    return LBM_OK;
+}
+
 // ------------------------------------------------------------------ domain
 
 struct lbm_domain {

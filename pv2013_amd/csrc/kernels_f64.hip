@@ -2,6 +2,8 @@
 // conservation reductions.  Built with -ffp-contract=off: every expression
 // rounds as written, which is what makes the FP64 path bit-identical to the
 // reference arithmetic.
+#include <algorithm>
+
 #include "step_kernels.h"
 
 using namespace lbm;
@@ -20,6 +22,28 @@ hipError_t launch_advance_iter(int64_t* iter, int64_t by, hipStream_t s) {
 hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s) {
     if (!iter) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_set_iter, dim3(1), dim3(1), 0, s, iter, value);
+    return hipGetLastError();
+}
+
+// Streaming probe for the measured-bandwidth figure: grid-stride float4 pass,
+// in place (b == nullptr: a[i] += 1, so no store can be dropped) or a copy.
+__global__ __launch_bounds__(256) void k_stream_probe(float4* a, float4* b, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float4 v = a[i];
+        if (b) {
+            b[i] = v;
+        } else {
+            v.x += 1.f; v.y += 1.f; v.z += 1.f; v.w += 1.f;
+            a[i] = v;
+        }
+    }
+}
+
+hipError_t launch_stream_probe(float4* a, float4* b, int64_t n, hipStream_t s) {
+    if (!a || n <= 0) return hipErrorInvalidValue;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_stream_probe, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n);
     return hipGetLastError();
 }
 

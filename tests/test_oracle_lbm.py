@@ -110,3 +110,20 @@ def test_periodic_shear_wave_decays_at_the_lattice_viscosity(oracle, q):
     ratio = (u[0] @ ux0) / (ux0 @ ux0)
     assert abs(ratio - np.exp(-nu * kk * kk * steps)) < 2e-3
     assert np.abs(u[1:]).max() < 1e-12
+
+
+@pytest.mark.parametrize("q,quirks", [(19, 1), (19, 0), (27, 1)])
+def test_threaded_step_equals_sequential(oracle, q, quirks):
+    """The OpenMP step (CPU baseline) gives the bits of the sequential step:
+    pulsatile inlet, pressure outlet, walls, both quirk modes."""
+    dims = (14, 14, 30)
+    cl, bcs = pipe_case(oracle, dims, 5.5, 0.04, waveform=0)
+    a, phi = np.array([0.3, 0.1]), np.array([0.2, 1.3])
+    sims = []
+    for _ in range(2):
+        s = oracle.Sim(q, dims, cl, 0.7, [oracle.make_bc(**b) for b in bcs], quirks=quirks)
+        s.set_waveform(0, 1, 50.0, a, phi)
+        sims.append(s)
+    sims[0].step(40)
+    sims[1].step_threads(40, 4)
+    assert np.array_equal(sims[0].f(), sims[1].f())

@@ -32,7 +32,7 @@ EXPORTS = (
     "lbm_domain_destroy", "lbm_domain_info", "lbm_set_waveform", "lbm_step", "lbm_step_timed",
     "lbm_sync", "lbm_iteration", "lbm_get_f", "lbm_set_f", "lbm_get_macros", "lbm_get_wss",
     "lbm_reduce", "lbm_get_macros_box", "lbm_get_wss_box", "lbm_batch_create", "lbm_autotune",
-    "lbm_domain_get_stats",
+    "lbm_domain_get_stats", "lbm_ctx_bandwidth",
 )
 
 

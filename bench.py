@@ -30,9 +30,9 @@ SEED = 2013
 PERIOD = 4000.0
 
 
-def run(P, ctx, case, storage, steps, warmup, scheme=0, options=0):
+def run(P, ctx, case, storage, steps, warmup, scheme=0, options=0, block=256):
     from pv2013_amd.cases import make_domain
-    dom = make_domain(ctx, case, storage=storage, scheme=scheme, options=options)
+    dom = make_domain(ctx, case, storage=storage, scheme=scheme, options=options, block=block)
     st = dom.stats()
     n = st["nodes"]
     dom.step(warmup)
@@ -119,6 +119,8 @@ def main():
                     help="skip the FP64 / FP16 comparison runs and the accuracy protocol")
     ap.add_argument("--scheme", default="aa", choices=["two", "aa"],
                     help="streaming: AA (one lattice, default) or two-lattice pull; identical results")
+    ap.add_argument("--block", type=int, default=256, choices=[64, 128, 256, 512],
+                    help="threads per block of the step kernels")
     ap.add_argument("--layout", default="fused", choices=["fused", "split", "generic"],
                     help="step kernel: records + row segments in one launch (default) or two, "
                          "or the generic per-node kernel")
@@ -139,7 +141,7 @@ def main():
           file=sys.stderr, flush=True)
     scheme = P.SCHEME_AA if args.scheme == "aa" else P.SCHEME_TWO_LATTICE
     options = {"fused": 0, "split": P.capi.OPT_SPLIT_LAUNCH, "generic": P.capi.OPT_GENERIC_KERNEL}[args.layout]
-    r = run(P, ctx, case, storage, args.steps, args.warmup, scheme=scheme, options=options)
+    r = run(P, ctx, case, storage, args.steps, args.warmup, scheme=scheme, options=options, block=args.block)
     n = r["nodes"]
     ms_step = r["ms"] / args.steps
     mlups = n * args.steps / (r["ms"] * 1e-3) / 1e6
@@ -149,7 +151,7 @@ def main():
              "wall_s_timed": r["wall_s"], "mass_after": r["mass"], "facets": case["ntri"],
              "voxelize_ms": case["vox_ms"], "classify_ms": case["cls_ms"], "scheme": args.scheme,
              "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3),
-             "layout": args.layout, "boundary_records": r["stats"]["boundary_records"],
+             "layout": args.layout, "block": args.block, "boundary_records": r["stats"]["boundary_records"],
              "row_chunks" if args.layout == "fused" else "segments": r["stats"]["segments"],
              "lane_use": ((n if args.layout == "fused" else n - r["stats"]["boundary_records"])
                           / max(1, 64 * r["stats"]["segments"])),
@@ -165,7 +167,8 @@ def main():
             if name == args.storage:
                 continue
             print("bench: comparison run %s" % name, file=sys.stderr, flush=True)
-            rr = run(P, ctx, case, st, min(args.steps, 1000), min(args.warmup, 100), scheme=scheme, options=options)
+            rr = run(P, ctx, case, st, min(args.steps, 1000), min(args.warmup, 100), scheme=scheme, options=options,
+                     block=args.block)
             extra["mlups_" + name] = rr["nodes"] * min(args.steps, 1000) / (rr["ms"] * 1e-3) / 1e6
         if "mlups_f64" in extra:
             extra["speedup_vs_f64"] = mlups / extra["mlups_f64"]

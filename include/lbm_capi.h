@@ -137,7 +137,7 @@ typedef struct {
     int32_t storage;  /* LBM_STORE_* */
     int32_t scheme;   /* LBM_SCHEME_* */
     int32_t quirks;   /* 1: the reference's step semantics; 0: clean bounce-back */
-    int32_t block;    /* threads per block, 0 = default */
+    int32_t block;    /* step-kernel threads per block: 64, 128, 256 or 512; 0 = 256 */
     double tau;
     const uint8_t* flags;    /* host, dims product */
     const lbm_bc_desc* bcs;  /* bcs[b] drives flag 3+b */
@@ -230,11 +230,14 @@ typedef struct {
     int32_t ntried;     /* candidates timed (0 when read from the cache) */
     double mlups;       /* fluid-node MLUPS of the best candidate */
     uint64_t key;       /* case signature used in the cache */
+    int32_t block;      /* best threads per block (lbm_domain_desc.block) */
+    int32_t pad_;
 } lbm_tune_result;
 
 /* Times each candidate layout of `desc` (streaming scheme two-lattice / AA,
- * node-order fast axis x / y / z, kernel: records + segments in one launch /
- * in two / generic) for `steps` steps
+ * node-order fast axis x / y / z, kernel: row chunks in one launch / records
+ * and segments in two / generic; then 128 / 256 / 512 threads per block for
+ * the winner) for `steps` steps
  * after a warm-up and returns the fastest.  When cache_path is not NULL the
  * result is looked up there first (key: lattice, storage, dims, flag hash) and
  * appended after tuning.  No reference counterpart. */

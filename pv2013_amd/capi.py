@@ -75,7 +75,8 @@ class DomainStats(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("fast_axis", C.c_int32), ("options", C.c_int32),
-                ("ntried", C.c_int32), ("mlups", C.c_double), ("key", C.c_uint64)]
+                ("ntried", C.c_int32), ("mlups", C.c_double), ("key", C.c_uint64), ("block", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class BatchCase(C.Structure):
@@ -410,10 +411,10 @@ def batch(ctx, cases, stack_axis=1):
 
 def autotune(ctx, steps=200, cache_path=None, **domain_kw):
     """Fastest layout for a case (lbm_autotune).  Returns a dict with scheme,
-    fast_axis, options, mlups, ntried and the cache key."""
+    fast_axis, options, block, mlups, ntried and the cache key."""
     d, keep = domain_desc(**domain_kw)
     r = TuneResult()
     ctx.check(lib().lbm_autotune(ctx.h, C.byref(d), steps,
                                  cache_path.encode() if cache_path else None, C.byref(r)), "lbm_autotune")
-    return {"scheme": r.scheme, "fast_axis": r.fast_axis, "options": r.options, "mlups": r.mlups,
-            "ntried": r.ntried, "key": "%016x" % r.key}
+    return {"scheme": r.scheme, "fast_axis": r.fast_axis, "options": r.options, "block": r.block,
+            "mlups": r.mlups, "ntried": r.ntried, "key": "%016x" % r.key}

@@ -277,6 +277,7 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     std::memset(&a, 0, sizeof a);
     a.n = D->n;
     a.ns = D->ns;
+    a.block = D->block;
     a.nrec = D->nrec;
     a.q = D->q;
     a.pnode = D->pnode;
@@ -380,6 +381,10 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     D->quirks = d.quirks;
     D->tau = d.tau;
     D->block = d.block > 0 ? d.block : 256;
+    if (D->block != 64 && D->block != 128 && D->block != 256 && D->block != 512) {
+        delete D;
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "block must be 64, 128, 256 or 512 threads");
+    }
     D->options = d.options;
     for (int a = 0; a < 3; ++a) D->dims[a] = d.dims[a];
     const int nx = d.dims[0], ny = d.dims[1], nz = d.dims[2];
@@ -1322,13 +1327,14 @@ extern "C" int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps
     if (cache_path) {
         if (FILE* fp = std::fopen(cache_path, "r")) {
             unsigned long long k;
-            int sch, ax, opt;
+            int sch, ax, opt, blk;
             double ml;
-            while (std::fscanf(fp, "%llx %d %d %d %lf", &k, &sch, &ax, &opt, &ml) == 5) {
+            while (std::fscanf(fp, "%llx %d %d %d %d %lf", &k, &sch, &ax, &opt, &blk, &ml) == 6) {
                 if (k == key) {
                     out->scheme = sch;
                     out->fast_axis = ax;
                     out->options = opt;
+                    out->block = blk;
                     out->mlups = ml;
                     std::fclose(fp);
                     return LBM_OK;
@@ -1338,39 +1344,49 @@ extern "C" int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps
         }
     }
     double best = -1.0;
+    // one candidate: build, warm up for a graph chunk, time `steps` steps
+    auto trial = [&](int sch, int ax, int opts, int blk) -> int {
+        lbm_domain_desc d = *desc;
+        d.scheme = sch;
+        d.fast_axis = ax;
+        d.options = opts;
+        d.block = blk;
+        lbm_domain* D = nullptr;
+        int rc = lbm_domain_create(ctx, &d, &D);
+        if (rc) return rc;
+        double ms = 0.0;
+        rc = lbm_step(D, 64);
+        if (!rc) rc = lbm_step_timed(D, steps, &ms);
+        const int64_t n = D->n;
+        lbm_domain_destroy(D);
+        if (rc) return rc;
+        const double mlups = ms > 0 ? (double)n * steps / (ms * 1e-3) / 1e6 : 0.0;
+        out->ntried++;
+        if (mlups > best) {
+            best = mlups;
+            out->scheme = sch;
+            out->fast_axis = ax;
+            out->options = opts;
+            out->block = blk;
+            out->mlups = mlups;
+        }
+        return LBM_OK;
+    };
+    // stage 1: scheme x node order x kernel layout at 256 threads per block
+    static const int kern_opt[3] = {0, LBM_OPT_SPLIT_LAUNCH, LBM_OPT_GENERIC_KERNEL};
+    const int keep = desc->options & ~(LBM_OPT_NO_ROW_FASTPATH | LBM_OPT_GENERIC_KERNEL | LBM_OPT_SPLIT_LAUNCH);
     for (int sch = 0; sch < 2; ++sch)
         for (int ax = 0; ax < 3; ++ax)
-            for (int kern = 0; kern < 3; ++kern) {
-                // records + segments in one launch / in two / generic kernel
-                static const int kern_opt[3] = {0, LBM_OPT_SPLIT_LAUNCH, LBM_OPT_GENERIC_KERNEL};
-                lbm_domain_desc d = *desc;
-                d.scheme = sch;
-                d.fast_axis = ax;
-                d.options = (desc->options & ~(LBM_OPT_NO_ROW_FASTPATH | LBM_OPT_GENERIC_KERNEL | LBM_OPT_SPLIT_LAUNCH)) |
-                            kern_opt[kern];
-                lbm_domain* D = nullptr;
-                int rc = lbm_domain_create(ctx, &d, &D);
-                if (rc) return rc;
-                double ms = 0.0;
-                rc = lbm_step(D, 64);
-                if (!rc) rc = lbm_step_timed(D, steps, &ms);
-                const int64_t n = D->n;
-                lbm_domain_destroy(D);
-                if (rc) return rc;
-                const double mlups = ms > 0 ? (double)n * steps / (ms * 1e-3) / 1e6 : 0.0;
-                out->ntried++;
-                if (mlups > best) {
-                    best = mlups;
-                    out->scheme = sch;
-                    out->fast_axis = ax;
-                    out->options = d.options;
-                    out->mlups = mlups;
-                }
-            }
+            for (int kern = 0; kern < 3; ++kern)
+                if (int rc = trial(sch, ax, keep | kern_opt[kern], 256)) return rc;
+    // stage 2: block size for the winner
+    const int bs = out->scheme, ba = out->fast_axis, bo = out->options;
+    for (int blk : {128, 512})
+        if (int rc = trial(bs, ba, bo, blk)) return rc;
     if (cache_path) {
         if (FILE* fp = std::fopen(cache_path, "a")) {
-            std::fprintf(fp, "%llx %d %d %d %.6f\n", (unsigned long long)key, out->scheme, out->fast_axis,
-                         out->options, out->mlups);
+            std::fprintf(fp, "%llx %d %d %d %d %.6f\n", (unsigned long long)key, out->scheme, out->fast_axis,
+                         out->options, out->block, out->mlups);
             std::fclose(fp);
         }
     }

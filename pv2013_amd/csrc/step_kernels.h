@@ -565,7 +565,7 @@ __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::c
 // path shares no addressing with the record/segment kernel below, and the
 // parity tests run both.
 template <int Q, class P, int MODE, int SCH>
-__global__ __launch_bounds__(256) void k_step(StepArgs a) {
+__global__ __launch_bounds__(512) void k_step(StepArgs a) {
     using L = Lat<Q>;
     using S = typename P::store_t;
     using T = typename P::calc_t;
@@ -648,7 +648,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 // and interior nodes alike, in node order; blocks map to XCDs in contiguous
 // ranges (logical_block).  Two launches (PART_RECORDS, then PART_SEGMENTS):
 // per XCD (block id mod 8) a contiguous share of record blocks or of segment
-// blocks (4 segments per 256-thread block).
+// blocks (one segment per wavefront).
 enum { PART_FUSED = 0, PART_RECORDS = 1, PART_SEGMENTS = 2 };
 
 // One boundary record per lane.  Every population has one source, chosen per
@@ -772,22 +772,22 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
 }
 
 template <int Q, class P, int MODE, int SCH, int PART>
-__global__ __launch_bounds__(256) void k_step_rs(StepArgs a) {
+__global__ __launch_bounds__(512) void k_step_rs(StepArgs a) {
     if constexpr (PART == PART_FUSED) {
         const int64_t lb = logical_block();
         if (lb >= a.nblocks) return;
-        const int64_t c = __builtin_amdgcn_readfirstlane((int)(lb * 4 + (threadIdx.x >> 6)));
+        const int64_t c = __builtin_amdgcn_readfirstlane((int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6)));
         if (c < a.nchunk) chunk_node<Q, P, MODE, SCH>(a, c, (int)(threadIdx.x & 63));
         return;
     }
     const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
     const int rb = PART == PART_SEGMENTS ? 0 : a.rb8;
     if (PART != PART_SEGMENTS && j < rb) {
-        const int64_t r = ((int64_t)xcd * rb + j) * 256 + threadIdx.x;
+        const int64_t r = ((int64_t)xcd * rb + j) * blockDim.x + threadIdx.x;
         if (r < a.nrec) record_node<Q, P, MODE, SCH>(a, r);
     } else if (PART != PART_RECORDS) {
         const int64_t lb = (int64_t)xcd * a.sb8 + (j - rb);
-        const int64_t s = __builtin_amdgcn_readfirstlane((int)(lb * 4 + (threadIdx.x >> 6)));
+        const int64_t s = __builtin_amdgcn_readfirstlane((int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6)));
         if (s < a.nseg) segment_node<Q, P, MODE, SCH>(a, s, (int)(threadIdx.x & 63));
     }
 }
@@ -823,6 +823,7 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a) {
 // returned to the caller, never a launch.
 inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
     if (!a.gin || !a.faces || !a.wvals || a.wk < 0 || a.wk >= LBM_WCHUNK || a.ns < a.n || (a.ns & 63)) return false;
+    if (a.block != 64 && a.block != 128 && a.block != 256 && a.block != 512) return false;
     if (layout == LBM_LAYOUT_GENERIC && (!a.pnode || !a.map || !a.bbits || !a.bpre)) return false;
     if (layout == LBM_LAYOUT_SPLIT && ((a.nseg > 0 && !a.seg) || (a.nrec > 0 && (!a.rsrc || !a.rec_node))))
         return false;
@@ -840,25 +841,26 @@ inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
 template <int Q, class P, int MODE, int SCH>
 void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
     if (layout == LBM_LAYOUT_GENERIC) {
-        hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, b);
+        hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, b.block)), dim3(b.block), 0, s, b);
     } else if (layout == LBM_LAYOUT_FUSED) {
         if constexpr (SCH == SCH_AA_LOCAL) {
             // The AA local step touches only each node's own slots: one dense
             // launch in node order, fully coalesced, record lanes fix up in place.
             StepArgs c = b;
             c.wreg = nullptr;
-            c.nblocks = (int)((b.n + 255) / 256);
-            hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, 256)), dim3(256), 0, s, c);
+            c.nblocks = (int)((b.n + b.block - 1) / b.block);
+            hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, b.block)), dim3(b.block), 0, s, c);
         } else {
-            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(b.nchunk, 4)), dim3(256),
+            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(b.nchunk, b.block / 64)),
+                               dim3(b.block),
                                0, s, b);
         }
     } else {
         // records first, then segments: the two launches write disjoint nodes
         if (b.rb8 > 0)
-            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_RECORDS>), dim3(8 * b.rb8), dim3(256), 0, s, b);
+            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_RECORDS>), dim3(8 * b.rb8), dim3(b.block), 0, s, b);
         if (b.sb8 > 0)
-            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_SEGMENTS>), dim3(8 * b.sb8), dim3(256), 0, s, b);
+            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_SEGMENTS>), dim3(8 * b.sb8), dim3(b.block), 0, s, b);
     }
 }
 
@@ -876,9 +878,10 @@ template <int Q, class P>
 hipError_t launch_step_t(int mode, int sch, int layout, const StepArgs& a, hipStream_t s) {
     if (!step_args_ok(mode, sch, layout, a)) return hipErrorInvalidValue;
     StepArgs b = a;
-    b.nblocks = (int)(layout == LBM_LAYOUT_FUSED ? (a.nchunk + 3) / 4 : (a.n + 255) / 256);
-    b.rb8 = (int)(((a.nrec + 255) / 256 + 7) / 8);
-    b.sb8 = (int)(((a.nseg + 3) / 4 + 7) / 8);
+    const int wpb = a.block / 64;  // wavefronts per block
+    b.nblocks = (int)(layout == LBM_LAYOUT_FUSED ? (a.nchunk + wpb - 1) / wpb : (a.n + a.block - 1) / a.block);
+    b.rb8 = (int)(((a.nrec + a.block - 1) / a.block + 7) / 8);
+    b.sb8 = (int)(((a.nseg + wpb - 1) / wpb + 7) / 8);
     if (mode == LBM_MODE_STEP) launch_sch<Q, P, LBM_MODE_STEP>(sch, layout, b, s);
     else if (mode == LBM_MODE_SNAP) launch_sch<Q, P, LBM_MODE_SNAP>(sch, layout, b, s);
     else launch_layout<Q, P, LBM_MODE_PRE, SCH_TWO>(layout, b, s);

@@ -261,3 +261,42 @@ def test_periodic_shear_wave(P, oracle, ctx, q, scheme):
     _, u = ref.macros()
     kk = 2 * np.pi / dims[1]
     assert abs((u[0] @ ux0) / (ux0 @ ux0) - np.exp(-(tau - 0.5) / 3 * kk * kk * steps)) < 2e-3
+
+
+def test_autotune_picks_a_valid_layout_and_caches_it(P, oracle, ctx, tmp_path):
+    """lbm_autotune times every candidate (scheme x node order x kernel
+    layout, then block sizes for the winner), returns one that builds and
+    reproduces the FP64 oracle bitwise, and answers from its cache file the
+    second time."""
+    dims = (16, 16, 40)
+    cl, bcs = pipe_case(oracle, dims, 6.0, 0.03)
+    cache = str(tmp_path / "tune.txt")
+    kw = dict(dims=dims, flags=cl, tau=0.8, bcs=[P.make_bc(**b) for b in bcs], q=19, storage=P.STORE_F64)
+    r1 = P.autotune(ctx, steps=20, cache_path=cache, **kw)
+    assert r1["ntried"] == 2 * 3 * 3 + 2 and r1["mlups"] > 0
+    assert r1["block"] in (128, 256, 512) and r1["scheme"] in (0, 1) and r1["fast_axis"] in (0, 1, 2)
+    r2 = P.autotune(ctx, steps=20, cache_path=cache, **kw)
+    assert r2["ntried"] == 0 and r2["key"] == r1["key"]
+    assert (r2["scheme"], r2["fast_axis"], r2["options"], r2["block"]) == \
+        (r1["scheme"], r1["fast_axis"], r1["options"], r1["block"])
+    dom = P.Domain(ctx, dims, cl, 0.8, kw["bcs"], scheme=r1["scheme"], fast_axis=r1["fast_axis"],
+                   options=r1["options"], block=r1["block"])
+    ref = oracle.Sim(19, dims, cl, 0.8, [oracle.make_bc(**b) for b in bcs], quirks=1)
+    dom.step(130)
+    ref.step(130)
+    assert bits_equal(dom.f(), ref.f())
+
+
+@pytest.mark.parametrize("block", [64, 128, 512])
+def test_block_sizes_give_identical_bits(P, oracle, ctx, block):
+    dims = (10, 11, 12)
+    cl, bcs = six_face_box(oracle, dims)
+    for options in LAYOUTS:
+        dom, ref = both(P, oracle, ctx, 19, dims, cl, 0.7, bcs, options=options)
+        alt = P.Domain(ctx, dims, cl, 0.7, [P.make_bc(**b) for b in bcs], q=19, options=options, block=block)
+        f0 = near_eq_state(oracle, 19, cl, np.random.default_rng(3))
+        dom.set_f(f0)
+        alt.set_f(f0)
+        dom.step(9)
+        alt.step(9)
+        assert bits_equal(dom.f(), alt.f()), "block %d options %d" % (block, options)

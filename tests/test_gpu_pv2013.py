@@ -1,0 +1,78 @@
+"""The reference's own case end to end on the GPU: PV2013.stl voxelized and
+classified at spacing 0.1 (ComplexGeometry.cpp:40-638), the inlet and six
+outlets of main()/Simulation (LBM.cpp:59-71, Boundary.cpp:9,
+Simulation.cpp:952-1348: u_in = (0, -0.1, 0) on the -y-moving face, outlets at
+rho = 1.01, the x-face outlets O3-O5 with the N_xz f10/f16 typo), tau = 0.66,
+FP64 with the reference's quirks, against the CPU oracle bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def pv2013_bcs():
+    # kind 0 velocity / 1 pressure; sigma: sign of c[axis] of the unknown populations
+    return [dict(kind=0, axis=1, sigma=-1, u=(0.0, -0.1, 0.0)),   # inlet, unknowns {3,7,11,16,17}
+            dict(kind=1, axis=1, sigma=-1, rho=1.01),              # O1, y-max face
+            dict(kind=1, axis=1, sigma=1, rho=1.01),               # O2, y-min face
+            dict(kind=1, axis=0, sigma=1, rho=1.01, typo=1),       # O3, x-min face
+            dict(kind=1, axis=0, sigma=1, rho=1.01, typo=1),       # O4
+            dict(kind=1, axis=0, sigma=1, rho=1.01, typo=1),       # O5
+            dict(kind=1, axis=2, sigma=1, rho=1.01)]               # O6, z-min face
+
+
+@pytest.fixture(scope="module")
+def P():
+    import pv2013_amd
+    return pv2013_amd
+
+
+@pytest.fixture(scope="module")
+def case(oracle):
+    import pv2013_amd as P
+    from test_gpu_voxelizer import to_capi_planes
+    z = np.load(os.path.join(ROOT, "tests", "golden", "pv2013.npz"))
+    tri = z["tri"]
+    ctx = P.Context(0)
+    g = P.grid_from_bbox(tri, 0.1)
+    flags, _ = ctx.voxelize(tri, g)
+    cl, _ = ctx.classify(g, flags, to_capi_planes(oracle.pv2013_planes()))
+    dims = tuple(int(d) for d in g.dims)
+    print("PV2013 box", dims, "fluid", int((cl > 0).sum()), flush=True)
+    yield ctx, dims, cl
+    ctx.close()
+
+
+def test_pv2013_fp64_matches_oracle(P, oracle, case):
+    ctx, dims, cl = case
+    bcs = pv2013_bcs()
+    dom = P.Domain(ctx, dims, cl, 0.66, [P.make_bc(**b) for b in bcs], q=19, storage=P.STORE_F64,
+                   scheme=P.SCHEME_AA)
+    ref = oracle.Sim(19, dims, cl, 0.66, [oracle.make_bc(**b) for b in bcs], quirks=1)
+    dom.step(20)
+    ref.step(20)
+    r1, u1 = dom.macros()
+    r2, u2 = ref.macros()
+    eq = lambda a, b: np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert eq(r1, r2) and eq(u1, u2)
+    assert eq(dom.wss(), ref.wss())
+    m, flux = dom.reduce()
+    assert abs(m - r2.sum()) < 1e-9 * m
+
+
+def test_pv2013_fp32_close_to_fp64(P, case):
+    ctx, dims, cl = case
+    bcs = [P.make_bc(**b) for b in pv2013_bcs()]
+    fl = cl > 0
+    out = {}
+    for st in (P.STORE_F64, P.STORE_F32):
+        dom = P.Domain(ctx, dims, cl, 0.66, bcs, q=19, storage=st, scheme=P.SCHEME_AA)
+        dom.step(2000)
+        out[st] = dom.macros()[1][:, fl]
+        dom.close()
+    err = np.linalg.norm(out[P.STORE_F32] - out[P.STORE_F64]) / np.linalg.norm(out[P.STORE_F64])
+    print("PV2013 FP32 velocity L2 vs FP64 after 2000 steps: %.3e" % err, flush=True)
+    assert err < 1e-5, err

@@ -522,7 +522,9 @@ __device__ __forceinline__ void apply_bc(const StepArgs& a, int64_t r, int type,
 // Pre-state output (modes PRE / SNAP), collision and store of node n.
 // dst(d) (AA gather only): destination node of population d, -1 if none;
 // evaluated at the store so no per-direction index stays live in registers.
-template <int Q, class P, int MODE, int SCH, bool BUF = true, class Dst>
+// ALLDST: every destination exists (segments, record-free chunks), so the
+// stores need no per-lane test.
+template <int Q, class P, int MODE, int SCH, bool BUF = true, bool ALLDST = false, class Dst>
 __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::calc_t* f, Dst dst) {
     using L = Lat<Q>;
     using S = typename P::store_t;
@@ -547,7 +549,9 @@ __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::c
             stp<S, BUF>(gout, L::opp(d), a.ns, n, to_store<S>(g[d]));
         } else {
             const int32_t t = dst(d);
-            if constexpr (BUF) {
+            if constexpr (ALLDST) {
+                stp<S, BUF>(gout, d, a.ns, t, to_store<S>(g[d]));
+            } else if constexpr (BUF) {
                 if (t >= 0) stp<S, BUF>(gout, d, a.ns, t, to_store<S>(g[d]));
                 else stp<S, BUF>(gout, L::opp(d), a.ns, n, to_store<S>(g[d]));
             } else {
@@ -710,7 +714,8 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
             f[d] = ldp<P>(gin, (SCH == SCH_AA_GATHER ? L::opp(d) : d), a.ns, row[2 + d] + lane);
         }
     }
-    emit<Q, P, MODE, SCH>(a, n, f, [&](int d) { return d == L::rest ? (int32_t)n : row[2 + L::opp(d)] + lane; });
+    emit<Q, P, MODE, SCH, true, true>(a, n, f,
+                                      [&](int d) { return d == L::rest ? (int32_t)n : row[2 + L::opp(d)] + lane; });
 }
 
 // One node of a row chunk (lane l = node start + l).  A chunk without records
@@ -732,14 +737,20 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     const int64_t NS = a.ns;
     const int64_t n = (int64_t)start + lane;
     const uint32_t mlo = (uint32_t)h[4], mhi = (uint32_t)h[5];
+    // the chunk row, read once into scalar registers (the stores below may
+    // not be reordered with later table reads)
+    int32_t hb[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) hb[d] = h[6 + 2 * d];
     T f[Q];
     if ((mlo | mhi) == 0u) {
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
             if (SCH == SCH_AA_LOCAL || d == L::rest) f[d] = ldp<P>(gin, d, NS, n);
-            else f[d] = ldp<P>(gin, SCH == SCH_AA_GATHER ? L::opp(d) : d, NS, (int64_t)h[6 + 2 * d] + lane);
+            else f[d] = ldp<P>(gin, SCH == SCH_AA_GATHER ? L::opp(d) : d, NS, (int64_t)hb[d] + lane);
         }
-        emit<Q, P, MODE, SCH>(a, n, f, [&](int d) { return d == L::rest ? (int32_t)n : h[6 + 2 * L::opp(d)] + lane; });
+        emit<Q, P, MODE, SCH, true, true>(a, n, f,
+                                          [&](int d) { return d == L::rest ? (int32_t)n : hb[L::opp(d)] + lane; });
         return;
     }
     const bool isrec = lane < 32 ? (mlo >> lane) & 1u : (mhi >> (lane - 32)) & 1u;

@@ -456,7 +456,11 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
                     map[p] = map[w];
                 }
     }
-    if (D->n >= (int64_t)INT32_MAX / 2) {
+    // ns (n rounded up to 64) stays below 2^30: FP32/FP16 kernels address a
+    // population array with 32-bit byte offsets and size its buffer
+    // descriptor in 32 bits (ns * 4 < 2^32), and offset 2^31 lies past every
+    // FP16 array.
+    if (D->n > ((int64_t)1 << 30) - 64) {
         delete D;
         return lbm_ctx_fail(ctx, LBM_ERANGE, "too many fluid nodes");
     }

@@ -737,11 +737,15 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     const int64_t NS = a.ns;
     const int64_t n = (int64_t)start + lane;
     const uint32_t mlo = (uint32_t)h[4], mhi = (uint32_t)h[5];
-    // the chunk row, read once into scalar registers (the stores below may
-    // not be reordered with later table reads)
-    int32_t hb[Q];
+    // The whole chunk row, read once into scalar registers before any store:
+    // a table read after a store could not be proved unclobbered and would
+    // become a per-lane vector load with a wait of its own.
+    int32_t hb[Q], hl[Q];
 #pragma unroll
-    for (int d = 0; d < Q; ++d) hb[d] = h[6 + 2 * d];
+    for (int d = 0; d < Q; ++d) {
+        hb[d] = h[6 + 2 * d];
+        hl[d] = h[7 + 2 * d];
+    }
     T f[Q];
     if ((mlo | mhi) == 0u) {
 #pragma unroll
@@ -758,6 +762,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     const uint32_t bb = isrec ? a.bbb[r] : 0u;
     const uint32_t lmiss = (SCH == SCH_AA_LOCAL && isrec) ? a.bmiss[r] : 0u;
     const S* stale = (const S*)a.bstale;
+    uint32_t pm = 0u;  // directions whose source node exists for this lane
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         const S* st = stale + (int64_t)d * a.nrec + r;
@@ -765,9 +770,9 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
         if (SCH == SCH_AA_LOCAL || d == L::rest) {
             p = ((lmiss & ~bb) >> d & 1u) ? st : gin + (int64_t)d * NS + n;
         } else {
-            const int32_t lh = h[7 + 2 * d];
-            const bool pres = lane >= (lh & 0xff) && lane < ((lh >> 8) & 0xff);
-            const S* nb = gin + (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * NS + ((int64_t)h[6 + 2 * d] + lane);
+            const bool pres = lane >= (hl[d] & 0xff) && lane < ((hl[d] >> 8) & 0xff);
+            pm |= (uint32_t)pres << d;
+            const S* nb = gin + (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * NS + ((int64_t)hb[d] + lane);
             const S* own = gin + (int64_t)(SCH == SCH_TWO ? L::opp(d) : d) * NS + n;
             p = pres ? nb : ((bb >> d & 1u) ? own : st);
         }
@@ -777,8 +782,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     // AA gather: the node at x + c_d is the source of direction opp d
     emit<Q, P, MODE, SCH, false>(a, n, f, [&](int d) {
         if (d == L::rest) return (int32_t)n;
-        const int32_t lh = h[7 + 2 * L::opp(d)];
-        return (lane >= (lh & 0xff) && lane < ((lh >> 8) & 0xff)) ? h[6 + 2 * L::opp(d)] + lane : -1;
+        return (pm >> L::opp(d)) & 1u ? hb[L::opp(d)] + lane : -1;
     });
 }
 

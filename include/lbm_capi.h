@@ -166,6 +166,11 @@ typedef struct {
 #define LBM_OPT_PERIODIC_Z 16
 #define LBM_OPT_GENERIC_KERNEL 32  /* generic per-node kernel instead of records + segments */
 #define LBM_OPT_SPLIT_LAUNCH 64    /* records and segments in two launches per step */
+/* FP16 row-chunk kernel: chunks holding boundary records address the lattice
+ * through 64-bit pointers instead of one 32-bit buffer descriptor over all Q
+ * arrays.  Automatic for FP16 lattices of 4 GiB or more; as an option it lets
+ * tests compare the two record paths.  Results are identical. */
+#define LBM_OPT_WIDE_RECORDS 128
 
 /* SubDomain(Geometry&) (Subdomain.cpp:3-33: CreateNodes, Connectivity,
  * Initialize, GetFluidHaloLat) + Simulation's constructor set-up

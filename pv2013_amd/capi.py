@@ -20,6 +20,7 @@ OPT_NO_GRAPH = 2
 OPT_PERIODIC_X, OPT_PERIODIC_Y, OPT_PERIODIC_Z = 4, 8, 16
 OPT_GENERIC_KERNEL = 32  # one thread per node through the site map
 OPT_SPLIT_LAUNCH = 64  # records and row segments in two launches
+OPT_WIDE_RECORDS = 128  # FP16 chunks with records through 64-bit addresses (automatic >= 4 GiB)
 BC_VELOCITY, BC_PRESSURE = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 WAVE_NONE, WAVE_FOURIER, WAVE_PV2013 = 0, 1, 2

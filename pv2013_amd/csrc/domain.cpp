@@ -278,6 +278,9 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.n = D->n;
     a.ns = D->ns;
     a.block = D->block;
+    // one 32-bit buffer descriptor spans the lattice only below 4 GiB
+    a.wide = (D->options & LBM_OPT_WIDE_RECORDS) ||
+             (uint64_t)D->q * (uint64_t)D->ns * store_bytes(D->storage) >= ((uint64_t)1 << 32);
     a.nrec = D->nrec;
     a.q = D->q;
     a.pnode = D->pnode;

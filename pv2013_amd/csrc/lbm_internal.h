@@ -38,6 +38,7 @@ struct StepArgs {
     int32_t nblocks;        // logical blocks (grid is rounded up to a multiple of 8)
     int32_t q;
     int32_t block;          // threads per block of the step kernels: 64, 128, 256 or 512
+    int32_t wide;           // 16-bit row chunks with records: 64-bit addresses (lattice >= 4 GiB)
     const int32_t* pnode;   // padded map index of node n
     const int32_t* map;     // padded site -> node id or -1
     const uint64_t* bbits;  // bit n: node n has a boundary record

@@ -16,6 +16,7 @@
 // last step is recovered by the same pull + BC without the collision (mode PRE).
 #pragma once
 
+#include <type_traits>
 #include "lattice.h"
 #include "lbm_internal.h"
 
@@ -410,6 +411,32 @@ __device__ __forceinline__ void stp(S* g, int d, int64_t N, int64_t i, S v) {
     }
 }
 
+// One buffer descriptor over a whole lattice, all Q population arrays of
+// stride N; element e of the lattice is array e / N, node e % N.  Only for
+// lattices below 4 GiB (the descriptor's size and the byte offsets are 32-bit).
+template <class S>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t lat_rsrc(const void* g, int q, int64_t N) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(g), (short)0,
+                                             (int)(uint32_t)((uint64_t)q * (uint64_t)N * sizeof(S)), 0x00020000);
+}
+template <class S>
+__device__ __forceinline__ uint32_t lat_ld(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+    if constexpr (sizeof(S) == 4) return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 4u), 0, 0);
+    else return __builtin_amdgcn_raw_buffer_load_b16(r, (int)(e * 2u), 0, 0);
+}
+template <class S>
+__device__ __forceinline__ void lat_st(__amdgpu_buffer_rsrc_t r, uint32_t e, S v) {
+    if constexpr (sizeof(S) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)(e * 4u), 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)(e * 2u), 0, 0);
+}
+template <class S, class T>
+__device__ __forceinline__ T from_bits(uint32_t b) {
+    if constexpr (sizeof(S) == 4) return (T)__builtin_bit_cast(float, b);
+    else return (T)__builtin_bit_cast(_Float16, (unsigned short)b);
+}
+
 // Conversion of a computed population to the storage type.  FP16 storage
 // rounds the FP32 result (the collision's last operation is an FMA rounded to
 // FP32) to FP16.  The compiler would otherwise fuse that FMA with the
@@ -523,8 +550,9 @@ __device__ __forceinline__ void apply_bc(const StepArgs& a, int64_t r, int type,
 // dst(d) (AA gather only): destination node of population d, -1 if none;
 // evaluated at the store so no per-direction index stays live in registers.
 // ALLDST: every destination exists (segments, record-free chunks), so the
-// stores need no per-lane test.
-template <int Q, class P, int MODE, int SCH, bool BUF = true, bool ALLDST = false, class Dst>
+// stores need no per-lane test.  LAT: each store picks its element of the
+// whole lattice (32-bit, one descriptor), no per-lane branch.
+template <int Q, class P, int MODE, int SCH, bool BUF = true, bool ALLDST = false, bool LAT = false, class Dst>
 __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::calc_t* f, Dst dst) {
     using L = Lat<Q>;
     using S = typename P::store_t;
@@ -549,7 +577,11 @@ __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::c
             stp<S, BUF>(gout, L::opp(d), a.ns, n, to_store<S>(g[d]));
         } else {
             const int32_t t = dst(d);
-            if constexpr (ALLDST) {
+            if constexpr (LAT) {
+                const uint32_t nS = (uint32_t)a.ns;
+                const uint32_t e = t >= 0 ? (uint32_t)d * nS + (uint32_t)t : (uint32_t)L::opp(d) * nS + (uint32_t)n;
+                lat_st<S>(lat_rsrc<S>(a.gout, Q, a.ns), e, to_store<S>(g[d]));
+            } else if constexpr (ALLDST) {
                 stp<S, BUF>(gout, d, a.ns, t, to_store<S>(g[d]));
             } else if constexpr (BUF) {
                 if (t >= 0) stp<S, BUF>(gout, d, a.ns, t, to_store<S>(g[d]));
@@ -725,7 +757,13 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
 // on the lanes of [lo_d, hi_d), else the node's own slot (bounce-back) or its
 // stale value, and issues one load; record lanes find their record index from
 // the chunk's record mask.
-template <int Q, class P, int MODE, int SCH>
+//
+// LAT (16-bit storage, lattice below 4 GiB): in a chunk with records each lane
+// picks, per direction, one 32-bit element index into the whole lattice (its
+// neighbour or its own slot) for one buffer load; stale values take a second
+// load only in chunks that have such lanes.  Otherwise the choice is among
+// 64-bit addresses, which the compiler turns into per-lane branches.
+template <int Q, class P, int MODE, int SCH, bool LAT = false>
 __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lane) {
     using L = Lat<Q>;
     using S = typename P::store_t;
@@ -763,6 +801,52 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     const uint32_t lmiss = (SCH == SCH_AA_LOCAL && isrec) ? a.bmiss[r] : 0u;
     const S* stale = (const S*)a.bstale;
     uint32_t pm = 0u;  // directions whose source node exists for this lane
+    if constexpr (LAT) {
+        static_assert(sizeof(S) != 8 && SCH != SCH_AA_LOCAL, "lattice descriptor path: 16/32-bit, pull or gather");
+        const uint32_t nS = (uint32_t)NS;
+        const __amdgpu_buffer_rsrc_t lat = lat_rsrc<S>(gin, Q, NS);
+#pragma unroll
+        for (int d = 0; d < Q; ++d)
+            if (d != L::rest) {
+                // lo <= lane < hi as one unsigned compare
+                const uint32_t lo = (uint32_t)hl[d] & 0xffu, hi = ((uint32_t)hl[d] >> 8) & 0xffu;
+                pm |= (uint32_t)((uint32_t)lane - lo < hi - lo) << d;
+            }
+        constexpr uint32_t nonrest = (uint32_t)((1ull << Q) - 1ull) & ~(1u << L::rest);
+        const uint32_t stn = isrec ? (~pm & ~bb & nonrest) : 0u;  // links that take the stale value
+        const bool anyst = __builtin_amdgcn_ballot_w64(stn != 0u) != 0ull;  // wave-uniform
+        auto gather = [&](auto with_stale) {
+#pragma unroll
+            for (int d = 0; d < Q; ++d) {
+                uint32_t e;
+                if (d == L::rest) {
+                    e = (uint32_t)d * nS + (uint32_t)n;
+                } else {
+                    const uint32_t enb =
+                        (uint32_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * nS + (uint32_t)(hb[d] + lane);
+                    const uint32_t eown = (uint32_t)(SCH == SCH_TWO ? L::opp(d) : d) * nS + (uint32_t)n;
+                    e = (pm >> d & 1u) ? enb : eown;
+                }
+                uint32_t bits = lat_ld<S>(lat, e);
+                if constexpr (decltype(with_stale)::value) {
+                    if (d != L::rest) {
+                        const bool us = stn >> d & 1u;
+                        const uint32_t sv = lat_ld<S>(pop_rsrc(stale, d, a.nrec), us ? (uint32_t)r : 0u);
+                        bits = us ? sv : bits;
+                    }
+                }
+                f[d] = from_bits<S, T>(bits);
+            }
+        };
+        if (anyst) gather(std::true_type{});
+        else gather(std::false_type{});
+        if (isrec) apply_bc<Q, P>(a, r, a.btype[r], f);
+        emit<Q, P, MODE, SCH, true, false, true>(a, n, f, [&](int d) {
+            if (d == L::rest) return (int32_t)n;
+            return (pm >> L::opp(d)) & 1u ? hb[L::opp(d)] + lane : -1;
+        });
+        return;
+    }
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         const S* st = stale + (int64_t)d * a.nrec + r;
@@ -786,13 +870,13 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     });
 }
 
-template <int Q, class P, int MODE, int SCH, int PART>
+template <int Q, class P, int MODE, int SCH, int PART, bool LAT = false>
 __global__ __launch_bounds__(512) void k_step_rs(StepArgs a) {
     if constexpr (PART == PART_FUSED) {
         const int64_t lb = logical_block();
         if (lb >= a.nblocks) return;
         const int64_t c = __builtin_amdgcn_readfirstlane((int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-        if (c < a.nchunk) chunk_node<Q, P, MODE, SCH>(a, c, (int)(threadIdx.x & 63));
+        if (c < a.nchunk) chunk_node<Q, P, MODE, SCH, LAT>(a, c, (int)(threadIdx.x & 63));
         return;
     }
     const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
@@ -865,10 +949,18 @@ void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
             c.wreg = nullptr;
             c.nblocks = (int)((b.n + b.block - 1) / b.block);
             hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, b.block)), dim3(b.block), 0, s, c);
+        } else if constexpr (sizeof(typename P::store_t) == 2) {
+            // 16-bit storage: chunks with records through one lattice-wide
+            // descriptor while the lattice is below 4 GiB
+            if (b.wide)
+                hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED, false>),
+                                   dim3(lbm_grid_blocks(b.nchunk, b.block / 64)), dim3(b.block), 0, s, b);
+            else
+                hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED, true>),
+                                   dim3(lbm_grid_blocks(b.nchunk, b.block / 64)), dim3(b.block), 0, s, b);
         } else {
             hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(b.nchunk, b.block / 64)),
-                               dim3(b.block),
-                               0, s, b);
+                               dim3(b.block), 0, s, b);
         }
     } else {
         // records first, then segments: the two launches write disjoint nodes

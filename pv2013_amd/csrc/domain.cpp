@@ -605,12 +605,15 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
         }
         if (D->nseg == 0) segs.assign(LBM_SEG_W, 0);
     }
-    // Fused layout: row chunks.  A chunk grows node by node while the next
-    // node follows the last one along the fast axis inside the same 64-node
-    // word and, for every direction, its source either is missing or continues
-    // the run of consecutive sources the chunk has so far (a direction whose
-    // sources stopped may not start again).  So per direction the lanes with a
-    // source form one interval [lo, hi) reading base + lane.
+    // Fused layout: row chunks.  A chunk grows node by node, up to 64, while
+    // the next node follows the last one along the fast axis and, for every
+    // direction, its source either is missing or continues the run of
+    // consecutive sources the chunk has so far (a direction whose sources
+    // stopped may not start again).  So per direction the lanes with a source
+    // form one interval [lo, hi) reading base + lane.  Two-lattice chunks also
+    // end at 64-node words, so a full chunk's stores to its own nodes are
+    // aligned; AA chunks store to neighbour runs and may span two words.
+    const bool word_chunks = D->scheme == LBM_SCHEME_TWO_LATTICE;
     if (D->layout == LBM_LAYOUT_FUSED) {
         const int CW = lbm_chunk_w(q);
         std::vector<int32_t> cbase(q, 0), clo(q, -1), chi(q, -1);
@@ -620,7 +623,9 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             int len = 0;
             for (;;) {
                 const int64_t x = start + len;
-                if (x >= D->n || len == 64 || (len > 0 && ((x & 63) == 0 || pnode[x] != pnode[x - 1] + 1))) break;
+                if (x >= D->n || len == 64 ||
+                    (len > 0 && ((word_chunks && (x & 63) == 0) || pnode[x] != pnode[x - 1] + 1)))
+                    break;
                 bool ok = true;
                 for (int e = 0; ok && len > 0 && e < q; ++e) {
                     if (e == rest) continue;
@@ -650,11 +655,15 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             const int sh = (int)(start & 63);
             h[0] = (int32_t)start;
             h[1] = len;
+            // records before the chunk; records are numbered in node order, so
+            // a lane's record index is h[2] + the mask bits below it
             h[2] = pre[start >> 6] + (sh ? __builtin_popcountll(word & ((1ull << sh) - 1ull)) : 0);
-            const uint64_t rm = (word >> sh) & (len == 64 ? ~0ull : ((1ull << len) - 1ull));
+            uint64_t rm = word >> sh;  // lane l <-> node start + l, across two words
+            if (sh && (start >> 6) + 1 < nwords) rm |= bits[(start >> 6) + 1] << (64 - sh);
+            rm &= len == 64 ? ~0ull : ((1ull << len) - 1ull);
             h[4] = (int32_t)(uint32_t)rm;
             h[5] = (int32_t)(uint32_t)(rm >> 32);
-            bool ok = len >= 1 && sh + len <= 64;
+            bool ok = len >= 1 && len <= 64 && (!word_chunks || sh + len <= 64);
             for (int e = 0; ok && e < q; ++e) {
                 if (e == rest) continue;
                 if (clo[e] >= 0) {

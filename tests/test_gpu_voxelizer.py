@@ -62,3 +62,29 @@ def test_random_tubes_match_bruteforce(ctx, oracle, seed):
         assert list(g.dims) == list(og.dims)
         flags, _ = ctx.voxelize(tri, g)
         assert np.array_equal(flags, oracle.voxelize(tri, og, 0))
+
+
+def test_c3_tube_blocks_match_oracle(ctx, oracle):
+    """The seeded tortuous-tube mesh of the voxelizer benchmark, at 256^3 on the
+    GPU: three sampled 16 x 16 column blocks equal the oracle's flags bit for
+    bit (the oracle voxelizes each block as its own grid with the full ray box)."""
+    from pv2013_amd import cases
+    n = 256
+    tri, g = cases.c3_mesh(seed=3, n=n)
+    print("c3 tube mesh", len(tri), "facets, grid", list(g.dims), flush=True)
+    assert tuple(g.origin) == (0.0, 0.0, 0.0) and g.spacing == 1.0
+    flags, _ = ctx.voxelize(tri, g)
+    vol = flags.reshape(n, n, n)
+    # two blocks centred on columns that cross the tube, one at random
+    cols = np.argwhere(vol.any(axis=0))  # (j, i) of columns holding fluid
+    rng = np.random.default_rng(0)
+    picks = [cols[k][::-1] - 8 for k in rng.choice(len(cols), 2, replace=False)]
+    picks.append(rng.integers(0, n - 16, 2))
+    fluid = 0
+    for i0, j0 in (np.clip(p, 0, n - 16).astype(int) for p in picks):
+        og = oracle.make_grid((float(i0), float(j0), 0.0), 1.0, (16, 16, n), tuple(g.raybox))
+        sub = oracle.voxelize(tri, og, 2).reshape(n, 16, 16)
+        mine = vol[:, j0:j0 + 16, i0:i0 + 16]
+        assert np.array_equal(sub, mine), (i0, j0)
+        fluid += int(sub.sum())
+    assert fluid > 1000

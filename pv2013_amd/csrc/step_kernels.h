@@ -507,7 +507,6 @@ __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64
     using L = Lat<Q>;
     using T = typename P::calc_t;
     const typename P::store_t* gin = (const typename P::store_t*)a.gin;
-    const int64_t N = a.n;
     const uint32_t bb = a.bbb[r];
     if (SCH == SCH_AA_LOCAL) miss = a.bmiss[r];
     // bounce-back: this node's own post-collision g[opp d] (SCH_AA_LOCAL:
@@ -559,7 +558,6 @@ __device__ __forceinline__ void emit(const StepArgs& a, int64_t n, typename P::c
     using T = typename P::calc_t;
     if (MODE == LBM_MODE_PRE || MODE == LBM_MODE_SNAP) write_pre<P, Q>(a, n, f);
     if (MODE == LBM_MODE_PRE) return;
-    const int64_t N = a.n;
     S* gout = (S*)a.gout;
     T g[Q];
     if constexpr (sizeof(T) == 8) {
@@ -735,7 +733,6 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
     const int32_t start = row[0], len = row[1];
     if (lane >= len) return;
     const S* gin = (const S*)a.gin;
-    const int64_t N = a.n;
     const int64_t n = (int64_t)start + lane;
     T f[Q];
 #pragma unroll

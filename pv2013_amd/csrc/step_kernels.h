@@ -867,8 +867,12 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     });
 }
 
+// D3Q27 with 16-bit storage: at most 96 VGPRs, five wavefronts per SIMD instead
+// of four (the compiler's own choice, 102 VGPRs; the fit needs no scratch).
+// FP32 and FP64 keep the default, whose extra registers they use.
 template <int Q, class P, int MODE, int SCH, int PART, bool LAT = false>
-__global__ __launch_bounds__(512) void k_step_rs(StepArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
+    Q == 27 && sizeof(typename P::store_t) == 2 ? 5 : 1))) void k_step_rs(StepArgs a) {
     if constexpr (PART == PART_FUSED) {
         const int64_t lb = logical_block();
         if (lb >= a.nblocks) return;

@@ -122,8 +122,8 @@ def main():
     ap.add_argument("--block", type=int, default=256, choices=[64, 128, 256, 512],
                     help="threads per block of the step kernels")
     ap.add_argument("--layout", default="fused", choices=["fused", "split", "generic"],
-                    help="step kernel: records + row segments in one launch (default) or two, "
-                         "or the generic per-node kernel")
+                    help="step kernel: row chunks in one launch (default), records + row segments "
+                         "in two, or the generic per-node kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the whole domain runs on a single MI355X "

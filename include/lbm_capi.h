@@ -147,14 +147,17 @@ typedef struct {
 } lbm_domain_desc;
 
 /* lbm_domain_desc.options */
-/* Step kernels.  Default: one launch per step in which boundary records (nodes
- * with a missing link or a boundary condition) take one thread each and every
- * other node belongs to a row segment (up to 64 consecutive node ids whose
- * sources per direction are consecutive ids) read with contiguous loads.
- * LBM_OPT_GENERIC_KERNEL: one thread per node, sources through the site map,
- * with a scalar fast path for whole interior wavefronts (off with
- * LBM_OPT_NO_ROW_FASTPATH, which implies the generic kernel).  Both give
- * identical results; they share no addressing code. */
+/* Step kernels.  Default: one launch per step over row chunks, one wavefront
+ * each: up to 64 consecutive node ids of one row whose sources per direction
+ * are consecutive ids, read with contiguous loads; the lanes of boundary
+ * records (nodes with a missing link or a boundary condition) pick their
+ * sources per lane.  The AA local step is one dense launch in node order.
+ * LBM_OPT_SPLIT_LAUNCH: records one thread each and row segments, in two
+ * launches.  LBM_OPT_GENERIC_KERNEL: one thread per node, sources through the
+ * site map, with a scalar fast path for whole interior wavefronts (off with
+ * LBM_OPT_NO_ROW_FASTPATH, which implies the generic kernel).  All give
+ * identical results; the generic kernel shares no addressing code with the
+ * others. */
 #define LBM_OPT_NO_ROW_FASTPATH 1  /* generic kernel, every lane through the site map */
 #define LBM_OPT_NO_GRAPH 2         /* launch every step eagerly instead of replaying a hipGraph */
 /* Periodic axes: the halo of the site map holds the wrapped node ids (a ghost
@@ -164,7 +167,7 @@ typedef struct {
 #define LBM_OPT_PERIODIC_X 4
 #define LBM_OPT_PERIODIC_Y 8
 #define LBM_OPT_PERIODIC_Z 16
-#define LBM_OPT_GENERIC_KERNEL 32  /* generic per-node kernel instead of records + segments */
+#define LBM_OPT_GENERIC_KERNEL 32  /* generic per-node kernel instead of row chunks */
 #define LBM_OPT_SPLIT_LAUNCH 64    /* records and segments in two launches per step */
 /* FP16 row-chunk kernel: chunks holding boundary records address the lattice
  * through 64-bit pointers instead of one 32-bit buffer descriptor over all Q
@@ -188,7 +191,7 @@ typedef struct {
     int64_t device_bytes;
     int32_t fast_axis;             /* node-order fastest axis */
     int32_t scheme;
-    int32_t layout;                /* 0 generic kernel, 1 records + segments in one launch, 2 in two */
+    int32_t layout;                /* 0 generic kernel, 1 row chunks (one launch), 2 records + segments (two) */
     int32_t pad_;
 } lbm_domain_stats;
 int lbm_domain_get_stats(const lbm_domain* dom, lbm_domain_stats* out);

@@ -686,9 +686,9 @@ __global__ __launch_bounds__(512) void k_step(StepArgs a) {
 enum { PART_FUSED = 0, PART_RECORDS = 1, PART_SEGMENTS = 2 };
 
 // One boundary record per lane.  Every population has one source, chosen per
-// lane without branches: the neighbour x - c_d when it exists, else this
-// node's own slot (bounce-back) or the stale value (unlinked populations the
-// reference never rewrites); then one load.  Zou-He runs on boundary-condition
+// lane: the neighbour x - c_d when it exists, else this node's own slot
+// (bounce-back) or the stale value (unlinked populations the reference never
+// rewrites); one load follows the choice.  Zou-He runs on boundary-condition
 // records only.
 template <int Q, class P, int MODE, int SCH>
 __device__ __forceinline__ void record_node(const StepArgs& a, int64_t r) {
@@ -750,10 +750,11 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
 // One node of a row chunk (lane l = node start + l).  A chunk without records
 // (the common case, a wave-uniform test) reads every direction as one
 // contiguous run base_d + l, exactly like a segment.  A chunk with records
-// picks, per lane and direction and without branches, the neighbour base_d + l
-// on the lanes of [lo_d, hi_d), else the node's own slot (bounce-back) or its
-// stale value, and issues one load; record lanes find their record index from
-// the chunk's record mask.
+// picks, per lane and direction, the neighbour base_d + l on the lanes of
+// [lo_d, hi_d), else the node's own slot (bounce-back) or its stale value, and
+// issues one load; record lanes find their record index from the chunk's
+// record mask (records are numbered in node order, so a chunk may span two
+// 64-node words).
 //
 // LAT (16-bit storage, lattice below 4 GiB): in a chunk with records each lane
 // picks, per direction, one 32-bit element index into the whole lattice (its

@@ -1,0 +1,90 @@
+"""The benchmark's full-size case: the 256x256x512 seeded Y-bifurcation with
+2.74M fluid nodes, built exactly as the benchmark builds it (voxelized and
+classified on the GPU).  The inlet is steady here, so no transcendental
+function is involved and the comparison can be bitwise.
+
+* FP64, AA and two-lattice: 70 steps (one 64-step graph replay and eager
+  steps) against the oracle on OpenMP threads, bit for bit: populations,
+  moments and wall shear stress.
+* FP32 and FP16 shifted storage: velocity within the stated bounds of the FP64
+  run after 600 steps (relative L2: 1e-5 for FP32, 1e-3 for FP16)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 70
+
+
+@pytest.fixture(scope="module")
+def P():
+    import pv2013_amd
+    return pv2013_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(P):
+    c = P.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def case(ctx):
+    from pv2013_amd import cases
+    print("building the 256x256x512 Y-bifurcation", flush=True)
+    c = cases.c2(ctx)
+    c["bcs"] = [dict(b, waveform=-1) for b in c["bcs"]]  # steady inlet
+    return c
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64))
+
+
+def domain(P, ctx, case, **kw):
+    return P.Domain(ctx, case["dims"], case["flags"], case["tau"], [P.make_bc(**b) for b in case["bcs"]],
+                    q=19, **kw)
+
+
+def test_fp64_full_size_bitwise(P, oracle, ctx, case):
+    threads = max(1, min(16, len(os.sched_getaffinity(0)), oracle.threads_available()))
+    ref = oracle.Sim(19, case["dims"], case["flags"], case["tau"], [oracle.make_bc(**b) for b in case["bcs"]],
+                     quirks=1)
+    print("oracle: %d fluid nodes, %d steps on %d threads" % (ref.nodes, STEPS, threads), flush=True)
+    ref.step_threads(STEPS, threads)
+    r_rho, r_u = ref.macros()
+    r_wss = ref.wss()
+    r_f = ref.f()
+    for scheme in (P.SCHEME_AA, P.SCHEME_TWO_LATTICE):
+        print("gpu FP64 scheme %d" % scheme, flush=True)
+        dom = domain(P, ctx, case, storage=P.STORE_F64, scheme=scheme)
+        assert dom.stats()["nodes"] == ref.nodes
+        dom.step(STEPS)
+        rho, u = dom.macros()
+        assert bits_equal(rho, r_rho) and bits_equal(u, r_u), "moments, scheme %d" % scheme
+        assert bits_equal(dom.wss(), r_wss), "wall shear stress, scheme %d" % scheme
+        f = dom.f()
+        assert bits_equal(f, r_f), "populations, scheme %d" % scheme
+        del f
+        dom.close()
+
+
+def test_low_precision_full_size(P, ctx, case):
+    from pv2013_amd import cases
+    steps = 600
+    ref = domain(P, ctx, case, storage=P.STORE_F64, scheme=P.SCHEME_AA)
+    ref.step(steps)
+    _, u64 = ref.macros()
+    ref.close()
+    fluid = case["flags"] > 0
+    for storage, bound in ((P.STORE_F32, 1e-5), (P.STORE_F16S, 1e-3)):
+        dom = domain(P, ctx, case, storage=storage, scheme=P.SCHEME_AA)
+        dom.step(steps)
+        _, u = dom.macros()
+        dom.close()
+        err = cases.errors(u64, u, fluid)
+        print("storage %d: velocity L2 %.3e (bound %.0e)" % (storage, err["l2"], bound), flush=True)
+        assert np.isfinite(u).all() and err["l2"] <= bound

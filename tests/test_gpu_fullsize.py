@@ -88,3 +88,29 @@ def test_low_precision_full_size(P, ctx, case):
         err = cases.errors(u64, u, fluid)
         print("storage %d: velocity L2 %.3e (bound %.0e)" % (storage, err["l2"], bound), flush=True)
         assert np.isfinite(u).all() and err["l2"] <= bound
+
+
+def test_d3q27_vessel_tree_full_size_bitwise(P, oracle, ctx):
+    """The D3Q27 benchmark geometry at full size (768x384x384 vessel tree,
+    6.32M fluid nodes, steady inlet): FP64 AA moments and wall shear stress
+    after 70 steps are bit-identical to the oracle.  (Dense populations would be
+    24 GB per copy; the moments and WSS are formed from all of them.)"""
+    from pv2013_amd import cases
+    print("building the 768x384x384 D3Q27 vessel tree", flush=True)
+    c = cases.c5(ctx)
+    threads = max(1, min(16, len(os.sched_getaffinity(0)), oracle.threads_available()))
+    ref = oracle.Sim(27, c["dims"], c["flags"], c["tau"], [oracle.make_bc(**b) for b in c["bcs"]], quirks=1)
+    print("oracle: %d fluid nodes, %d steps on %d threads" % (ref.nodes, STEPS, threads), flush=True)
+    ref.step_threads(STEPS, threads)
+    r_rho, r_u = ref.macros()
+    r_wss = ref.wss()
+    nodes = ref.nodes
+    del ref
+    dom = P.Domain(ctx, c["dims"], c["flags"], c["tau"], [P.make_bc(**b) for b in c["bcs"]], q=27,
+                   storage=P.STORE_F64, scheme=P.SCHEME_AA)
+    assert dom.stats()["nodes"] == nodes
+    dom.step(STEPS)
+    rho, u = dom.macros()
+    assert bits_equal(rho, r_rho) and bits_equal(u, r_u)
+    assert bits_equal(dom.wss(), r_wss)
+    dom.close()

@@ -18,11 +18,12 @@
  *                        PV2013 fixtures (every fixture point maps to a grid index).
  *   orc_voxelize         ComplexGeometry.cpp:264-357, 431-506.  Pinned by the
  *                        PV2013 fixtures (260,051 boundary points), by mode 0
- *                        (literal brute force) == modes 1/2 on random meshes, by
- *                        the reference's own labelPoints built from its sources
- *                        (oracle/_ref, where it builds), and by closed shapes
- *                        (a box and a sphere voxelised against their analytic
- *                        inside test away from the surface).
+ *                        (literal brute force) == modes 1/2 on random meshes,
+ *                        and by closed shapes (a box and a sphere voxelised
+ *                        against their analytic inside test away from the
+ *                        surface).  The reference's labelPoints itself does not
+ *                        build here (ComplexGeometry.cpp includes mpi_call.h,
+ *                        whose template at :143 gcc and clang reject).
  *   orc_classify         ComplexGeometry.cpp:509-638, Subdomain.cpp:375-478.
  *                        Pinned by the PV2013 inlet/outlet/wall fixtures.
  *   collision            Simulation.cpp:21-179.  Pinned by: rest state is a fixed
@@ -47,6 +48,14 @@
  *   waveform             Simulation.cpp:875-900, pulsatile.h:12-28 (PV2013 mode);
  *                        the Fourier mode has no reference counterpart.  Pinned
  *                        against long-double evaluation.
+ *   orc_sim_step         Simulation.cpp:903-932: Collision, Streaming, inlet,
+ *                        wall, outlets, in that order, over the node lists.  Each
+ *                        part is pinned as above, and the whole by mass
+ *                        conservation, the resting box and Poiseuille flow.  The
+ *                        reference's Simulation.cpp does not build here (it needs
+ *                        the VTK library through OutputFile.h), so the time
+ *                        evolution of the quirk-laden step as a whole is parity
+ *                        unpinned beyond those checks.
  */
 #include "lbm_oracle.h"
 

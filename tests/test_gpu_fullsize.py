@@ -114,3 +114,19 @@ def test_d3q27_vessel_tree_full_size_bitwise(P, oracle, ctx):
     assert bits_equal(rho, r_rho) and bits_equal(u, r_u)
     assert bits_equal(dom.wss(), r_wss)
     dom.close()
+
+
+def test_aa_equals_two_lattice_full_size(P, ctx, case):
+    """FP32 and FP16 at full size: the AA scheme (row chunks that may span two
+    64-node words) and the two-lattice pull (chunks that end at words) give the
+    same moments bit for bit after 70 steps."""
+    for storage in (P.STORE_F32, P.STORE_F16S):
+        out = []
+        for scheme in (P.SCHEME_AA, P.SCHEME_TWO_LATTICE):
+            print("storage %d scheme %d" % (storage, scheme), flush=True)
+            dom = domain(P, ctx, case, storage=storage, scheme=scheme)
+            dom.step(STEPS)
+            out.append(dom.macros())
+            dom.close()
+        (r0, u0), (r1, u1) = out
+        assert bits_equal(r0, r1) and bits_equal(u0, u1), "storage %d" % storage

@@ -43,6 +43,8 @@ bool load_case(const std::string& path, CaseConfig& c, std::string& err) {
     c.output_every = (int)get("output_every", c.output_every);
     c.device = (int)get("device", 0);
     if (!getstr("output_dir").empty()) c.output_dir = getstr("output_dir");
+    c.autotune = get("autotune", 0) != 0;
+    c.autotune_cache = getstr("autotune_cache");
     const std::string st = getstr("storage");
     if (st == "f32") c.storage = LBM_STORE_F32;
     else if (st == "f16s" || st == "f16") c.storage = LBM_STORE_F16S;

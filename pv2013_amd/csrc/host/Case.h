@@ -32,6 +32,11 @@ struct CaseConfig {
     int output_every = 100;                 // Simulation.cpp:920
     std::string output_dir = ".";
     int device = 0;
+    // lbm_autotune before building the domain: scheme, node order, kernel
+    // layout and block size are timed and the fastest is used.  Every choice
+    // gives the same results bit for bit; only the speed changes.
+    bool autotune = false;
+    std::string autotune_cache;  // tuning cache file ("" = none)
 };
 
 bool load_case(const std::string& path, CaseConfig& out, std::string& err);

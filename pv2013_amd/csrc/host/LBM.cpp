@@ -64,6 +64,10 @@ int main(int argc, char** argv) {
         }
         std::printf("fluid nodes %lld, boundary records %lld\n", (long long)subdomain.fluid_nodes,
                     (long long)subdomain.boundary_records);
+        if (subdomain.tuned.ntried >= 0)
+            std::printf("autotune: scheme %d, fast axis %d, options %d, block %d (%d candidates, %.0f MLUPS)\n",
+                        subdomain.tuned.scheme, subdomain.tuned.fast_axis, subdomain.tuned.options,
+                        subdomain.tuned.block, subdomain.tuned.ntried, subdomain.tuned.mlups);
         Simulation sim(subdomain, cfg.tau, 1.0, 0.5e-5);
         if (!sim.Iterate(cfg.steps, cfg.output_every, cfg.output_dir)) {
             std::fprintf(stderr, "simulation: %s\n", sim.error.c_str());

@@ -20,6 +20,20 @@ SubDomain::SubDomain(Geometry& g, const CaseConfig& cfg, lbm_ctx* ctx) : Geometr
     d.bcs = cfg.bcs.data();
     d.nbc = (int32_t)cfg.bcs.size();
     d.fast_axis = -1;
+    if (cfg.autotune) {
+        lbm_tune_result t;
+        if (lbm_autotune(ctx, &d, 128, cfg.autotune_cache.empty() ? nullptr : cfg.autotune_cache.c_str(), &t) !=
+            LBM_OK) {
+            ok = false;
+            error = lbm_last_error(ctx);
+            return;
+        }
+        d.scheme = t.scheme;
+        d.fast_axis = t.fast_axis;
+        d.options = t.options;
+        d.block = t.block;
+        tuned = t;
+    }
     if (lbm_domain_create(ctx, &d, &dom) != LBM_OK) {
         ok = false;
         error = lbm_last_error(ctx);

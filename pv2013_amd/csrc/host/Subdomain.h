@@ -23,4 +23,6 @@ public:
     std::vector<int64_t> wall_sites;
     bool ok = true;
     std::string error;
+    // the autotuner's choice when the configuration asked for one (ntried = -1: not tuned)
+    lbm_tune_result tuned = {0, -1, 0, -1, 0.0, 0, 0, 0};
 };

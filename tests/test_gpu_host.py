@@ -54,3 +54,30 @@ def test_lbm_run_matches_python_path(tmp_path):
     k, rem = walls // (nx * ny), walls % (nx * ny)
     order = np.lexsort((k, rem // nx, rem % nx))  # i outermost, then j, then k
     assert np.array_equal(w_vtu, wss[walls[order]])
+
+
+def test_lbm_run_autotune_gives_identical_files(tmp_path):
+    """`autotune 1` in the configuration times the candidate layouts first and
+    runs the fastest; every layout gives the same bits, so the VTU files equal
+    those of the untuned run byte for byte.  A second run reads the choice
+    back from the tuning cache."""
+    cfg, _, _ = write_case(str(tmp_path), storage="f32")
+    base = open(cfg).read()
+    tuned_cfg = os.path.join(str(tmp_path), "tuned.cfg")
+    cache = os.path.join(str(tmp_path), "tune.cache")
+    with open(tuned_cfg, "w") as fh:
+        fh.write(base.replace(os.path.join(str(tmp_path), "out"), os.path.join(str(tmp_path), "out_tuned")))
+        fh.write("autotune 1\nautotune_cache %s\n" % cache)
+    print("lbm_run plain and tuned", flush=True)
+    r = subprocess.run([RUN, cfg], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for attempt in range(2):
+        r = subprocess.run([RUN, tuned_cfg], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        m = re.search(r"autotune: .*\((\d+) candidates", r.stdout)
+        assert m, r.stdout
+        assert (int(m.group(1)) > 0) == (attempt == 0), r.stdout  # the second run reads the cache
+        for f in ("Fluid_Variables/100_rank_0.vtu", "Wall_Shear_Stress_VTK/100_rank_0_wall_shear_stress.vtu"):
+            a = open(os.path.join(str(tmp_path), "out", f), "rb").read()
+            b = open(os.path.join(str(tmp_path), "out_tuned", f), "rb").read()
+            assert a == b, f

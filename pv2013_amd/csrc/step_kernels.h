@@ -13,7 +13,9 @@
 //          (Boundary.cpp:127-148, Simulation.cpp:952-1348)
 //   BGK    rho, u, f_eq, g_out = omega f_eq + (1 - omega) f (Simulation.cpp:21-179)
 // The stored g is the post-collision state; the reference's state f after the
-// last step is recovered by the same pull + BC without the collision (mode PRE).
+// last step is recovered by the same pull + BC without the collision (mode PRE,
+// two lattices) or written by the last step itself (mode SNAP, AA, whose single
+// lattice no longer holds the previous state).
 #pragma once
 
 #include <type_traits>
@@ -501,7 +503,7 @@ __device__ __forceinline__ void apply_bc(const StepArgs& a, int64_t r, int type,
 
 // Bounce-back, stale populations and Zou-He of boundary record r on the
 // gathered populations f (miss: links whose source node does not exist).
-template <int Q, class P, int SCH, bool BUF = true>
+template <int Q, class P, int SCH>
 __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64_t n, uint32_t miss,
                                              typename P::calc_t* f) {
     using L = Lat<Q>;
@@ -515,8 +517,8 @@ __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
             if (!((miss & bb) >> d & 1u)) continue;
-            if (SCH == SCH_TWO) f[d] = ldp<P, BUF>(gin, L::opp(d), a.ns, n);
-            if (SCH == SCH_AA_GATHER) f[d] = ldp<P, BUF>(gin, d, a.ns, n);
+            if (SCH == SCH_TWO) f[d] = ldp<P>(gin, L::opp(d), a.ns, n);
+            if (SCH == SCH_AA_GATHER) f[d] = ldp<P>(gin, d, a.ns, n);
         }
     }
     // unlinked populations the reference never rewrites keep their value

@@ -64,6 +64,57 @@ def test_random_tubes_match_bruteforce(ctx, oracle, seed):
         assert np.array_equal(flags, oracle.voxelize(tri, og, 0))
 
 
+def facet_soup(oracle, seed):
+    """A soup of facets whose file normals are not their geometric normals
+    (random, tiny z component, exactly axial, powers of two), many of them
+    placed so that an end of the t window falls on, or within rounding of, a
+    grid point of some column.  The grid extends beyond the mesh, the spacing
+    is 0.37 or 1 (integer vertices: rays through edges and vertices, offset
+    columns on the ray box), and the window is [-1e-7, 1 + 1e-7] or [0, 1].
+    Seeds 0 and 1 keep the moved vertices near the grid; seed 2 lets them go
+    anywhere, so the ray box's z extent (the ray length) is about 1e12.
+    Returns (tri, origin, spacing, dims, raybox, t_eps)."""
+    rng = np.random.default_rng(100 + seed)
+    n, dims = 400, (24, 20, 48)
+    sp = 1.0 if seed == 1 else 0.37
+    origin = np.array((-3.0, -3.0, -3.0) if seed == 1 else (-1.3, -0.7, -2.1))
+    teps = 0.0 if seed == 1 else 1e-7
+    ext = np.array(dims) * sp
+    tri = np.zeros((n, 12))
+    for v in range(3):
+        tri[:, 3 * v:3 * v + 3] = origin + rng.uniform(-0.1, 1.1, (n, 3)) * ext
+    if seed == 1:
+        tri[:, :9] = np.round(tri[:, :9])
+    kind = np.arange(n) % 5
+    nrm = rng.uniform(-1, 1, (n, 3))
+    nrm[kind == 1, 2] *= 1e-9
+    nrm[kind == 2] = [0.0, 0.0, 1.0]
+    nrm[kind == 3] = np.ldexp(1.0, rng.integers(-40, 3, (np.count_nonzero(kind == 3), 3)))
+    tri[:, 9:] = nrm
+    # the plane's height at one column on (or within 1e-13 of) a grid point
+    for f in np.nonzero(rng.uniform(size=n) < 0.5)[0]:
+        i, j, k = (int(rng.integers(0, d)) for d in dims)
+        sx, sy = origin[0] + i * sp, origin[1] + j * sp
+        nx, ny, nz = tri[f, 9:]
+        a = (tri[f, 0] - sx) * nx + (tri[f, 1] - sy) * ny
+        z = origin[2] + k * sp + rng.choice([0.0, 1e-13, -1e-13]) - a / nz
+        if seed == 2 or abs(z - origin[2] - 0.5 * ext[2]) < ext[2]:
+            tri[f, 2] = z
+    return tri, tuple(origin), sp, dims, tuple(oracle.bbox(tri)), teps
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_facet_soup_with_file_normals(ctx, oracle, seed):
+    """Flags of the facet soup equal the oracle's literal per-point count
+    (ComplexGeometry.cpp:284-330) bit for bit."""
+    import pv2013_amd as P
+    tri, origin, sp, dims, rb, teps = facet_soup(oracle, seed)
+    flags, _ = ctx.voxelize(tri, P.make_grid(origin, sp, dims, rb, t_eps=teps))
+    ref = oracle.voxelize(tri, oracle.make_grid(origin, sp, dims, rb, t_eps=teps), 0)
+    assert 0 < int(ref.sum()) < ref.size
+    assert np.array_equal(flags, ref)
+
+
 def test_c3_tube_blocks_match_oracle(ctx, oracle):
     """The seeded tortuous-tube mesh of the voxelizer benchmark, at 256^3 on the
     GPU: three sampled 16 x 16 column blocks equal the oracle's flags bit for

@@ -14,13 +14,14 @@
 //          column tiles by their XY bbox widened by a margin that bounds the
 //          rounding error of the sign test (so binning never loses a facet the
 //          exact test would accept);
-//   device one 64-lane workgroup per tile, one lane per column; facets staged
-//          through LDS 64 at a time; for each facet that passes the exact XY
-//          test the lane toggles the parity bits (held in LDS) of the grid
-//          points whose t is in the window: the reference's t expression is
-//          evaluated at the few points near the two ends of that interval and
-//          the points between them are toggled as a range (window_toggle);
-//          bits are written out as flags.
+//   device one workgroup of four wavefronts per tile (tiles with the longest
+//          lists launch first), one lane per column in each wavefront; the
+//          wavefronts take the tile's facets in turns of 64, staged through
+//          LDS; for each facet that passes the exact XY test the lane toggles
+//          the parity bits (held in LDS) of the grid points whose t is in the
+//          window: the reference's t expression is evaluated at the few points
+//          near the two ends of that interval and the points between them are
+//          toggled as a range (window_toggle); bits are written out as flags.
 // Built with -ffp-contract=off so t and the sign test round as the reference.
 #include <hip/hip_runtime.h>
 
@@ -40,12 +41,13 @@ int lbm_ctx_device(lbm_ctx* ctx);
 namespace {
 
 constexpr int TILE = 8;           // 8x8 columns per workgroup
+constexpr int VWAVES = 4;         // wavefronts per workgroup sharing a tile's facet list
 constexpr int MAXNZ = 2048;       // parity bits per column held in LDS
-constexpr int NZW = MAXNZ / 32;
 
 struct VoxArgs {
     const double* tri;        // 12 doubles per facet
     const int32_t* tile_off;  // ntiles + 1
+    const int32_t* order;     // tiles by decreasing list length, one per workgroup
     const int32_t* tile_tri;
     const int32_t* global_tri;
     int32_t nglobal;
@@ -80,19 +82,22 @@ __device__ __forceinline__ bool t_window(const VoxArgs& a, const double* T, doub
     return tt >= -a.teps && tt <= 1 + a.teps;
 }
 
+// Parity bits of the workgroup's 64 columns: word w of column c at
+// bits[w * 64 + c].  The workgroup's wavefronts toggle them concurrently, so
+// every toggle is an LDS atomic xor.
 __device__ __forceinline__ void toggle_range(uint32_t* bits, int tid, int lo, int hi) {
     for (int w = lo >> 5; w <= (hi >> 5); ++w) {
         uint32_t m = 0xffffffffu;
         if (w == (lo >> 5)) m &= 0xffffffffu << (lo & 31);
         if (w == (hi >> 5)) m &= 0xffffffffu >> (31 - (hi & 31));
-        bits[w * 64 + tid] ^= m;
+        atomicXor(&bits[w * 64 + tid], m);
     }
 }
 
 __device__ __forceinline__ void toggle_exact(const VoxArgs& a, const double* T, double sx, double sy, double ex,
                                              double ey, uint32_t* bits, int tid, int lo, int hi) {
     for (int k = lo; k <= hi; ++k)
-        if (t_window(a, T, sx, sy, ex, ey, k)) bits[(k >> 5) * 64 + tid] ^= 1u << (k & 31);
+        if (t_window(a, T, sx, sy, ex, ey, k)) atomicXor(&bits[(k >> 5) * 64 + tid], 1u << (k & 31));
 }
 
 // Toggles the parity bit of every k whose t lies in the window, for one facet
@@ -151,38 +156,45 @@ __device__ void window_toggle(const VoxArgs& a, const double* T, double sx, doub
     toggle_exact(a, T, sx, sy, ex, ey, bits, tid, max(b0, 0), min(b1, last));
 }
 
+// The workgroup's wavefronts take the list's batches of 64 facets in turn
+// (batch r * VWAVES + wave in round r), each staging its batch through its own
+// LDS slice; all threads run the same number of rounds, so the barriers are
+// uniform.
 __device__ void process_list(const VoxArgs& a, const int32_t* list, int count, double* sT,
                              uint32_t* bits, bool valid, double sx, double sy, double ex, double ey) {
-    const int tid = threadIdx.x;
-    for (int base = 0; base < count; base += 64) {
-        const int m = min(64, count - base);
+    const int tid = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rounds = (count + 64 * VWAVES - 1) / (64 * VWAVES);
+    double* mine = sT + wave * 12 * 64;
+    for (int r = 0; r < rounds; ++r) {
+        const int base = (r * VWAVES + wave) * 64;
+        const int m = max(0, min(64, count - base));
         __syncthreads();
         if (tid < m) {
             const double* T = a.tri + 12 * (int64_t)list[base + tid];
 #pragma unroll
-            for (int c = 0; c < 12; ++c) sT[c * 64 + tid] = T[c];
+            for (int c = 0; c < 12; ++c) mine[c * 64 + tid] = T[c];
         }
         __syncthreads();
         if (!valid) continue;
         for (int t = 0; t < m; ++t) {
             double T[12];
 #pragma unroll
-            for (int c = 0; c < 12; ++c) T[c] = sT[c * 64 + t];
+            for (int c = 0; c < 12; ++c) T[c] = mine[c * 64 + t];
             if (inside_xy(sx, sy, T)) window_toggle(a, T, sx, sy, ex, ey, bits, tid);
         }
     }
 }
 
-__global__ __launch_bounds__(64) void k_voxelize(VoxArgs a) {
-    __shared__ double sT[12 * 64];
-    __shared__ uint32_t bits[NZW * 64];
-    const int tid = threadIdx.x;
-    const int tile = blockIdx.x;
+__global__ __launch_bounds__(64 * VWAVES) void k_voxelize(VoxArgs a) {
+    __shared__ double sT[VWAVES * 12 * 64];
+    extern __shared__ uint32_t bits[];  // ceil(nz / 32) * 64 words
+    const int tid = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tile = a.order[blockIdx.x];
     const int tx = tile % a.ntx, ty = tile / a.ntx;
     const int i = tx * TILE + (tid % TILE), j = ty * TILE + (tid / TILE);
     const bool valid = i < a.nx && j < a.ny;
     const int nw = (a.nz + 31) >> 5;
-    for (int w = 0; w < nw; ++w) bits[w * 64 + tid] = 0u;
+    for (int w = wave; w < nw; w += VWAVES) bits[w * 64 + tid] = 0u;
     // the ray of ComplexGeometry.cpp:441-458 with adjustLineCoordinates case 0
     double sx = a.ox + i * a.sp, sy = a.oy + j * a.sp;
     double ex = sx, ey = sy;
@@ -197,7 +209,7 @@ __global__ __launch_bounds__(64) void k_voxelize(VoxArgs a) {
     if (!valid) return;
     const int64_t nxy = (int64_t)a.nx * a.ny;
     const int64_t col = i + (int64_t)a.nx * j;
-    for (int k = 0; k < a.nz; ++k)
+    for (int k = wave; k < a.nz; k += VWAVES)
         a.flags[col + nxy * k] = (uint8_t)((bits[(k >> 5) * 64 + tid] >> (k & 31)) & 1u);
 }
 
@@ -320,6 +332,11 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
     for (int64_t i = 0; i < ntiles; ++i) cnt[i + 1] += cnt[i];
     if (cnt[ntiles] > INT32_MAX / 2) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: bin lists too long");
     std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1), lists(std::max<int64_t>(cnt[ntiles], 1));
+    // longest lists first, so the slowest workgroups start first
+    std::vector<int32_t> order(ntiles);
+    for (int64_t i = 0; i < ntiles; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return cnt[x + 1] - cnt[x] > cnt[y + 1] - cnt[y]; });
     for (size_t u = 0; u < keep.size(); ++u) {
         const auto& r = rng[u];
         for (int ty = r[2]; ty <= r[3]; ++ty)
@@ -327,21 +344,23 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
     }
     DevMem m;
     double* d_tri;
-    int32_t *d_off, *d_list, *d_glob;
+    int32_t *d_off, *d_list, *d_glob, *d_order;
     uint8_t* d_flags;
     const int64_t nsites = (int64_t)nx * ny * nz;
     VCHECK(m.alloc(&d_tri, (size_t)std::max<int64_t>(ntri, 1) * 12));
     VCHECK(m.alloc(&d_off, (size_t)ntiles + 1));
+    VCHECK(m.alloc(&d_order, (size_t)ntiles));
     VCHECK(m.alloc(&d_list, lists.size()));
     VCHECK(m.alloc(&d_glob, std::max<size_t>(global.size(), 1)));
     VCHECK(m.alloc(&d_flags, (size_t)nsites));
     if (ntri > 0) VCHECK(hipMemcpyAsync(d_tri, tri, sizeof(double) * 12 * ntri, hipMemcpyHostToDevice, st));
     VCHECK(hipMemcpyAsync(d_off, cnt.data(), sizeof(int32_t) * (ntiles + 1), hipMemcpyHostToDevice, st));
+    VCHECK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * ntiles, hipMemcpyHostToDevice, st));
     VCHECK(hipMemcpyAsync(d_list, lists.data(), sizeof(int32_t) * lists.size(), hipMemcpyHostToDevice, st));
     if (!global.empty())
         VCHECK(hipMemcpyAsync(d_glob, global.data(), sizeof(int32_t) * global.size(), hipMemcpyHostToDevice, st));
     VoxArgs a;
-    a.tri = d_tri; a.tile_off = d_off; a.tile_tri = d_list; a.global_tri = d_glob;
+    a.tri = d_tri; a.tile_off = d_off; a.order = d_order; a.tile_tri = d_list; a.global_tri = d_glob;
     a.nglobal = (int32_t)global.size();
     a.nx = nx; a.ny = ny; a.nz = nz; a.ntx = ntx;
     a.ox = ox; a.oy = oy; a.oz = g->origin[2]; a.sp = sp;
@@ -353,7 +372,8 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
     VCHECK(hipEventCreate(&e0));
     VCHECK(hipEventCreate(&e1));
     VCHECK(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_voxelize, dim3((unsigned)ntiles), dim3(64), 0, st, a);
+    const size_t lds_bits = sizeof(uint32_t) * 64 * (size_t)((nz + 31) / 32);
+    hipLaunchKernelGGL(k_voxelize, dim3((unsigned)ntiles), dim3(64 * VWAVES), lds_bits, st, a);
     hipError_t le = hipGetLastError();
     VCHECK(hipEventRecord(e1, st));
     VCHECK(le);

@@ -10,7 +10,7 @@
 // The XY test does not depend on z, so the work is organised per column:
 //   host   facets with nz == 0 are dropped (their t is +-inf or NaN, never in
 //          the window); facets whose XY projection is thinner than ~1e-6 rad
-//          go to a list every column tests; the rest are binned into 8x8
+//          go to a list every column tests; the rest are binned into 8 x 8
 //          column tiles by their XY bbox widened by a margin that bounds the
 //          rounding error of the sign test (so binning never loses a facet the
 //          exact test would accept);
@@ -40,7 +40,8 @@ int lbm_ctx_device(lbm_ctx* ctx);
 
 namespace {
 
-constexpr int TILE = 8;           // 8x8 columns per workgroup
+constexpr int TX = 8, TY = 8;     // a tile of TX x TY columns per workgroup, one per lane
+static_assert(TX * TY == 64, "one column per lane of a wavefront");
 constexpr int VWAVES = 4;         // wavefronts per workgroup sharing a tile's facet list
 constexpr int MAXNZ = 2048;       // parity bits per column held in LDS
 
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(64 * VWAVES) void k_voxelize(VoxArgs a) {
     const int tid = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tile = a.order[blockIdx.x];
     const int tx = tile % a.ntx, ty = tile / a.ntx;
-    const int i = tx * TILE + (tid % TILE), j = ty * TILE + (tid / TILE);
+    const int i = tx * TX + (tid % TX), j = ty * TY + (tid / TX);
     const bool valid = i < a.nx && j < a.ny;
     const int nw = (a.nz + 31) >> 5;
     for (int w = wave; w < nw; w += VWAVES) bits[w * 64 + tid] = 0u;
@@ -304,7 +305,7 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
         ext = std::max(ext, std::fabs(g->raybox[2 * a + 1] - g->raybox[2 * a]) + std::fabs(g->origin[a]) +
                                 g->dims[a] * g->spacing);
     const double margin = 1e-7 * std::max(ext, 1.0);
-    const int ntx = (nx + TILE - 1) / TILE, nty = (ny + TILE - 1) / TILE;
+    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
     const int64_t ntiles = (int64_t)ntx * nty;
     // bin facets into column tiles
     std::vector<int32_t> keep, global;
@@ -322,8 +323,8 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
         i0 = std::max(i0, 0L); j0 = std::max(j0, 0L);
         i1 = std::min(i1, (long)nx - 1); j1 = std::min(j1, (long)ny - 1);
         if (i0 > i1 || j0 > j1) continue;
-        std::array<int32_t, 4> r = {(int32_t)(i0 / TILE), (int32_t)(i1 / TILE), (int32_t)(j0 / TILE),
-                                    (int32_t)(j1 / TILE)};
+        std::array<int32_t, 4> r = {(int32_t)(i0 / TX), (int32_t)(i1 / TX), (int32_t)(j0 / TY),
+                                    (int32_t)(j1 / TY)};
         keep.push_back((int32_t)t);
         rng.push_back(r);
         for (int ty = r[2]; ty <= r[3]; ++ty)

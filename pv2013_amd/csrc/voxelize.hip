@@ -307,6 +307,7 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
     const double margin = 1e-7 * std::max(ext, 1.0);
     const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
     const int64_t ntiles = (int64_t)ntx * nty;
+    if (ntiles > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: more than 2^31 column tiles");
     // bin facets into column tiles
     std::vector<int32_t> keep, global;
     std::vector<int32_t> cnt(ntiles + 1, 0);

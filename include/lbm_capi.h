@@ -205,7 +205,9 @@ int lbm_set_waveform(lbm_domain* dom, int slot, int kind, double period, int nk,
 /* Simulation::Iterate body (Simulation.cpp:908-918) nsteps times: collision,
  * streaming, inlet, wall, outlet.  Enqueued on the context stream. */
 int lbm_step(lbm_domain* dom, int64_t nsteps);
-/* Same, and returns the device time of the steps (HIP events on the stream). */
+/* Same, and returns the device time of the steps (HIP events on the stream).
+ * A step graph not yet built is captured before the first event, so the time
+ * holds the steps only. */
 int lbm_step_timed(lbm_domain* dom, int64_t nsteps, double* ms);
 int lbm_sync(lbm_domain* dom);
 int64_t lbm_iteration(const lbm_domain* dom);

@@ -220,27 +220,37 @@ struct ClsArgs {
     int32_t nx, ny, nz;
     int32_t nplanes;
     const int32_t* box;  // per plane: lo[3], hi[3] index intervals
+    int32_t bx;          // workgroups per x-row
 };
 
 // ComplexGeometry.cpp:509-556 (boundary: fluid with an outside 26-neighbour or
 // on a grid face), :579-638 (plane membership, here as index intervals worked
 // out on the host in long double) and Subdomain.cpp:413-473 (priority).
+// Workgroup b covers 256 sites of x-row b / bx (row = j + ny k), so the
+// indices come from one 32-bit division instead of 64-bit ones per site.
 __global__ __launch_bounds__(256) void k_classify(ClsArgs a) {
-    const int64_t nxy = (int64_t)a.nx * a.ny;
-    const int64_t ns = nxy * a.nz;
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
+    const uint32_t b = blockIdx.x;
+    const uint32_t row = b / (uint32_t)a.bx;
+    const int i = (int)(b - row * (uint32_t)a.bx) * 256 + (int)threadIdx.x;
+    if (i >= a.nx) return;
+    const int j = (int)(row % (uint32_t)a.ny), k = (int)(row / (uint32_t)a.ny);
+    const int64_t s = (int64_t)row * a.nx + i;
     const uint8_t v = a.in[s];
     if (v != 1) { a.out[s] = 0; return; }
-    const int i = (int)(s % a.nx), j = (int)((s / a.nx) % a.ny), k = (int)(s / nxy);
     bool bnd = (i == 0 || i == a.nx - 1 || j == 0 || j == a.ny - 1 || k == 0 || k == a.nz - 1);
-    for (int dz = -1; dz <= 1 && !bnd; ++dz)
-        for (int dy = -1; dy <= 1 && !bnd; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int x = i + dx, y = j + dy, z = k + dz;
-                if (x < 0 || x >= a.nx || y < 0 || y >= a.ny || z < 0 || z >= a.nz) continue;
-                if (a.in[x + a.nx * (y + (int64_t)a.ny * z)] == 0) { bnd = true; break; }
-            }
+    if (!bnd) {
+        // off the grid faces all 26 neighbours exist: issue their loads together
+        const int64_t sy = a.nx, sz = (int64_t)a.nx * a.ny;
+        uint32_t outside = 0u;
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx)
+                    if (dx != 0 || dy != 0 || dz != 0) outside |= a.in[s + dx + sy * dy + sz * dz] == 0;
+        bnd = outside != 0u;
+    }
     if (!bnd) { a.out[s] = 1; return; }
     uint8_t f = 2;
     for (int p = 0; p < a.nplanes; ++p) {
@@ -440,12 +450,14 @@ extern "C" int lbm_classify(lbm_ctx* ctx, const lbm_grid_desc* g, const lbm_plan
     VCHECK(m.alloc(&d_box, box.size()));
     VCHECK(hipMemcpyAsync(d_in, flags, (size_t)ns, hipMemcpyHostToDevice, st));
     VCHECK(hipMemcpyAsync(d_box, box.data(), sizeof(int32_t) * box.size(), hipMemcpyHostToDevice, st));
-    ClsArgs a{d_in, d_out, nx, ny, nz, nplanes, d_box};
+    const int64_t bx = (nx + 255) / 256, blocks = bx * ny * (int64_t)nz;
+    if (blocks > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_classify: grid too large");
+    ClsArgs a{d_in, d_out, nx, ny, nz, nplanes, d_box, (int32_t)bx};
     hipEvent_t e0, e1;
     VCHECK(hipEventCreate(&e0));
     VCHECK(hipEventCreate(&e1));
     VCHECK(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_classify, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)blocks), dim3(256), 0, st, a);
     hipError_t le = hipGetLastError();
     VCHECK(hipEventRecord(e1, st));
     VCHECK(le);

@@ -154,8 +154,8 @@ struct lbm_domain {
     WaveTab* d_waves = nullptr;
     int64_t* d_iter = nullptr;
     void* g[2] = {nullptr, nullptr};
-    int32_t *d_wall = nullptr, *d_bcnode = nullptr, *d_bcface = nullptr;
-    double *fpre = nullptr, *rho = nullptr, *u = nullptr, *wss = nullptr, *red = nullptr;
+    int32_t *d_wall = nullptr, *d_bcnode = nullptr, *d_bcface = nullptr, *d_bcstart = nullptr;
+    double *fpre = nullptr, *rho = nullptr, *u = nullptr, *wss = nullptr, *red = nullptr, *red_scratch = nullptr;
     int cur = 0;
     int64_t iter = 0;
     int state = 0;            // 0: f = w (initial), 1: fpre holds f, 2: recompute from g[cur ^ 1]
@@ -746,6 +746,24 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->d_bcnode, std::max<size_t>(D->bc_nodes.size(), 1)));
     DCHECK(D->alloc(&D->d_bcface, std::max<size_t>(D->bc_face.size(), 1)));
     DCHECK(D->alloc(&D->red, (size_t)LBM_MAXBC + 2));
+    DCHECK(D->alloc(&D->red_scratch, (size_t)lbm_reduce_scratch((int64_t)D->bc_nodes.size())));
+    // boundary-condition nodes grouped by condition, node order within one
+    std::vector<int32_t> bcstart(D->faces.size() + 1, 0);
+    {
+        std::vector<size_t> idx(D->bc_nodes.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return D->bc_face[x] < D->bc_face[y]; });
+        std::vector<int32_t> nodes(idx.size()), faces(idx.size());
+        for (size_t i = 0; i < idx.size(); ++i) {
+            nodes[i] = D->bc_nodes[idx[i]];
+            faces[i] = D->bc_face[idx[i]];
+            bcstart[faces[i] + 1]++;
+        }
+        for (size_t b = 0; b < D->faces.size(); ++b) bcstart[b + 1] += bcstart[b];
+        D->bc_nodes.swap(nodes);
+        D->bc_face.swap(faces);
+    }
+    DCHECK(D->alloc(&D->d_bcstart, bcstart.size()));
     if ((rc = upload(D, D->pnode, pnode.data(), sizeof(int32_t) * pnode.size()))) return fail(rc);
     if ((rc = upload(D, D->map, map.data(), sizeof(int32_t) * map.size()))) return fail(rc);
     if ((rc = upload(D, D->bbits, bits.data(), sizeof(uint64_t) * bits.size()))) return fail(rc);
@@ -769,6 +787,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if ((rc = upload(D, D->d_wall, D->wall_nodes.data(), sizeof(int32_t) * D->wall_nodes.size()))) return fail(rc);
     if ((rc = upload(D, D->d_bcnode, D->bc_nodes.data(), sizeof(int32_t) * D->bc_nodes.size()))) return fail(rc);
     if ((rc = upload(D, D->d_bcface, D->bc_face.data(), sizeof(int32_t) * D->bc_face.size()))) return fail(rc);
+    if ((rc = upload(D, D->d_bcstart, bcstart.data(), sizeof(int32_t) * bcstart.size()))) return fail(rc);
     const int64_t zero64 = 0;
     if ((rc = upload(D, D->d_iter, &zero64, sizeof zero64))) return fail(rc);
     // Node.cpp:30-37: f = w everywhere, so the stale populations hold w
@@ -1154,8 +1173,8 @@ extern "C" int lbm_reduce(lbm_domain* D, double* mass, double* flux) {
     CHECK(hipSetDevice(ctx->device));
     int rc = ensure_pre(D);
     if (rc) return rc;
-    CHECK(launch_reduce(D->rho, D->u, D->n, D->d_bcnode, D->d_bcface, (int64_t)D->bc_nodes.size(), D->d_faces,
-                        (int)D->faces.size(), D->red, ctx->stream));
+    CHECK(launch_reduce(D->rho, D->u, D->n, D->d_bcnode, D->d_bcface, D->d_bcstart, (int64_t)D->bc_nodes.size(),
+                        D->d_faces, (int)D->faces.size(), D->red_scratch, D->red, ctx->stream));
     std::vector<double> h(D->faces.size() + 2);
     CHECK(hipMemcpyAsync(h.data(), D->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
     CHECK(hipStreamSynchronize(ctx->stream));

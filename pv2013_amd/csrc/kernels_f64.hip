@@ -114,44 +114,87 @@ hipError_t launch_wss(int q, const WssArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// FP64 health check: out[0] = sum rho, out[1 + b] = sum rho u_axis over the
-// nodes of boundary condition b, out[1 + nbc] = number of non-finite rho.
-__global__ __launch_bounds__(256) void k_reduce(const double* rho, const double* u, int64_t n,
-                                                const int32_t* bnode, const int32_t* bface,
-                                                int64_t nbnd, const FaceTab* faces, int nbc,
-                                                double* out) {
-    __shared__ double part[256];
-    double m = 0.0, bad = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double r = rho[i];
-        if (isfinite(r)) m += r;
-        else bad += 1.0;
-    }
-    part[threadIdx.x] = m;
+// FP64 health check, summed in a fixed order so that every run and every
+// device gives the same bits:
+//   out[0]       = sum of the finite rho over the nodes,
+//   out[1 + b]   = sum of rho u_axis over the nodes of boundary condition b
+//                  (axis = its face normal),
+//   out[1 + nbc] = number of nodes whose rho is not finite.
+// Stage 1 (k_reduce_part).  Block g < LBM_RED_BLOCKS takes nodes
+// [g C, (g + 1) C) with C = 256 ceil(n / (256 LBM_RED_BLOCKS)); thread t adds
+// nodes g C + t, g C + t + 256, ... to 0.0 in turn, then the 256 thread sums
+// are folded as a tree (sum[t] += sum[t + h] for h = 128, 64, ..., 1).  Each
+// later block writes the flux term rho u_axis of 256 boundary nodes, which are
+// listed grouped by condition and in node order within one.
+// Stage 2 (k_reduce_final).  Block o sums its inputs (the stage-1 partials,
+// or the flux terms of one condition) in the same thread-then-tree order.
+__device__ __forceinline__ double tree_fold(double v, double* part) {
+    part[threadIdx.x] = v;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(&out[0], part[0]);
-    if (bad > 0.0) atomicAdd(&out[1 + nbc], bad);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbnd;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t x = bnode[i];
-        const int b = bface[i];
-        const int ax = faces[b].axis;
-        atomicAdd(&out[1 + b], rho[x] * u[(int64_t)ax * n + x]);
+    const double r = part[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_part(const double* rho, const double* u, int64_t n, int64_t chunk,
+                                                     const int32_t* bnode, const int32_t* bface, int64_t nbnd,
+                                                     const FaceTab* faces, double* scratch) {
+    __shared__ double part[256];
+    const int64_t g = blockIdx.x;
+    if (g < LBM_RED_BLOCKS) {
+        const int64_t lo = g * chunk, hi = lo + chunk < n ? lo + chunk : n;
+        double m = 0.0, bad = 0.0;
+        for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
+            const double r = rho[i];
+            if (isfinite(r)) m = m + r;
+            else bad = bad + 1.0;
+        }
+        m = tree_fold(m, part);
+        bad = tree_fold(bad, part);
+        if (threadIdx.x == 0) {
+            scratch[g] = m;
+            scratch[LBM_RED_BLOCKS + g] = bad;
+        }
+    } else {
+        const int64_t i = (g - LBM_RED_BLOCKS) * 256 + threadIdx.x;
+        if (i < nbnd) {
+            const int32_t x = bnode[i];
+            scratch[2 * LBM_RED_BLOCKS + i] = rho[x] * u[(int64_t)faces[bface[i]].axis * n + x];
+        }
     }
 }
 
+__global__ __launch_bounds__(256) void k_reduce_final(const double* scratch, const int32_t* bstart, int nbc,
+                                                      double* out) {
+    __shared__ double part[256];
+    const int o = blockIdx.x;
+    int64_t lo, hi;
+    if (o == 0) { lo = 0; hi = LBM_RED_BLOCKS; }
+    else if (o <= nbc) { lo = 2 * LBM_RED_BLOCKS + bstart[o - 1]; hi = 2 * LBM_RED_BLOCKS + bstart[o]; }
+    else { lo = LBM_RED_BLOCKS; hi = 2 * LBM_RED_BLOCKS; }
+    double v = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) v = v + scratch[i];
+    v = tree_fold(v, part);
+    if (threadIdx.x == 0) out[o] = v;
+}
+
 hipError_t launch_reduce(const double* rho, const double* u, int64_t n, const int32_t* bnode,
-                         const int32_t* bface, int64_t nbnd, const FaceTab* faces, int nbc,
-                         double* out, hipStream_t s) {
-    if (!rho || !u || !out || (nbnd > 0 && (!bnode || !bface || !faces))) return hipErrorInvalidValue;
-    hipError_t e = hipMemsetAsync(out, 0, sizeof(double) * (2 + nbc), s);
+                         const int32_t* bface, const int32_t* bstart, int64_t nbnd, const FaceTab* faces,
+                         int nbc, double* scratch, double* out, hipStream_t s) {
+    if (!rho || !u || !out || !scratch || !bstart || (nbnd > 0 && (!bnode || !bface || !faces)) || nbc < 0)
+        return hipErrorInvalidValue;
+    const int64_t per = (n + 256LL * LBM_RED_BLOCKS - 1) / (256LL * LBM_RED_BLOCKS);
+    const int64_t chunk = 256 * (per > 0 ? per : 1);
+    const int64_t blocks = LBM_RED_BLOCKS + (nbnd + 255) / 256;
+    if (blocks > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reduce_part, dim3((unsigned)blocks), dim3(256), 0, s, rho, u, n, chunk, bnode, bface, nbnd,
+                       faces, scratch);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_reduce, dim3(512), dim3(256), 0, s, rho, u, n, bnode, bface, nbnd, faces, nbc,
-                       out);
+    hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)(nbc + 2)), dim3(256), 0, s, scratch, bstart, nbc, out);
     return hipGetLastError();
 }

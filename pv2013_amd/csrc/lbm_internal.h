@@ -127,9 +127,14 @@ hipError_t launch_wave_chunk(const WaveTab* waves, const int32_t* used, int nuse
 hipError_t launch_step_f64(int mode, int sch, int layout, const StepArgs& a, hipStream_t s);
 hipError_t launch_init_f64(int q, bool swap, const InitArgs& a, hipStream_t s);
 hipError_t launch_wss(int q, const WssArgs& a, hipStream_t s);
+// FP64 reductions in a fixed order (kernels_f64.hip): bnode/bface list the
+// boundary-condition nodes grouped by condition (bstart[b] .. bstart[b + 1]),
+// scratch holds lbm_reduce_scratch(nbnd) doubles.
+constexpr int LBM_RED_BLOCKS = 1024;
+inline int64_t lbm_reduce_scratch(int64_t nbnd) { return 2 * LBM_RED_BLOCKS + (nbnd > 0 ? nbnd : 1); }
 hipError_t launch_reduce(const double* rho, const double* u, int64_t n, const int32_t* bnode,
-                         const int32_t* bface, int64_t nbnd, const FaceTab* faces, int nbc,
-                         double* out /* 1 + nbc + 1 */, hipStream_t s);
+                         const int32_t* bface, const int32_t* bstart, int64_t nbnd, const FaceTab* faces,
+                         int nbc, double* scratch, double* out /* 1 + nbc + 1 */, hipStream_t s);
 // kernels_lp.hip (storage LBM_STORE_F32 or LBM_STORE_F16S)
 hipError_t launch_step_lp(int storage, int mode, int sch, int layout, const StepArgs& a, hipStream_t s);
 hipError_t launch_init_lp(int storage, int q, bool swap, const InitArgs& a, hipStream_t s);

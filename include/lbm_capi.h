@@ -228,7 +228,9 @@ int lbm_get_wss_box(lbm_domain* dom, const int32_t origin[3], const int32_t dims
  * (Simulation.cpp:1374-1421), zero elsewhere. */
 int lbm_get_wss(lbm_domain* dom, double* wss);
 /* FP64 health check: total mass and per-boundary-condition mass flux through
- * its face (flux has nbc entries, may be NULL).  LBM_ENAN on a non-finite state. */
+ * its face (flux has nbc entries, may be NULL).  LBM_ENAN on a non-finite state.
+ * The sums run in a fixed order (no atomics): the same state gives the same
+ * bits on every call and every device. */
 int lbm_reduce(lbm_domain* dom, double* mass, double* flux);
 
 /* ---------------------------------------------------------------- autotune */

@@ -21,7 +21,7 @@
 //          the parity bits (held in LDS) of the grid points whose t is in the
 //          window: the reference's t expression is evaluated at the few points
 //          near the two ends of that interval and the points between them are
-//          toggled as a range (window_toggle); bits are written out as flags.
+//          toggled as a range (voxel_window.h); bits are written out as flags.
 // Built with -ffp-contract=off so t and the sign test round as the reference.
 #include <hip/hip_runtime.h>
 
@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "lbm_internal.h"
+#include "voxel_window.h"
 
 int lbm_ctx_fail(lbm_ctx* ctx, int code, const std::string& msg);
 hipStream_t lbm_ctx_stream(lbm_ctx* ctx);
@@ -71,91 +72,22 @@ __device__ __forceinline__ bool inside_xy(double x, double y, const double* T) {
     return !(neg && pos);
 }
 
-// The reference's t window for grid point k of this column, literally
-// (ComplexGeometry.cpp:441-458 then :315-316).
-__device__ __forceinline__ bool t_window(const VoxArgs& a, const double* T, double sx, double sy, double ex,
-                                         double ey, int k) {
-    const double nx = T[9], ny = T[10], nz = T[11];
-    const double sz = a.oz + k * a.sp;
-    const double ez = sz + a.L + a.pad;
-    const double tt = ((T[0] - sx) * nx + (T[1] - sy) * ny + (T[2] - sz) * nz) /
-                      ((ex - sx) * nx + (ey - sy) * ny + (ez - sz) * nz);
-    return tt >= -a.teps && tt <= 1 + a.teps;
-}
-
-// Parity bits of the workgroup's 64 columns: word w of column c at
-// bits[w * 64 + c].  The workgroup's wavefronts toggle them concurrently, so
-// every toggle is an LDS atomic xor.
-__device__ __forceinline__ void toggle_range(uint32_t* bits, int tid, int lo, int hi) {
-    for (int w = lo >> 5; w <= (hi >> 5); ++w) {
-        uint32_t m = 0xffffffffu;
-        if (w == (lo >> 5)) m &= 0xffffffffu << (lo & 31);
-        if (w == (hi >> 5)) m &= 0xffffffffu >> (31 - (hi & 31));
-        atomicXor(&bits[w * 64 + tid], m);
+// Parity bits of the workgroup's 64 columns in LDS: word w of column c at
+// bits[w * 64 + c].  The workgroup's wavefronts flip them concurrently, so
+// every flip is an LDS atomic xor.
+struct LdsColumnBits {
+    uint32_t* bits;
+    int col;
+    __device__ void flip(int k) { atomicXor(&bits[(k >> 5) * 64 + col], 1u << (k & 31)); }
+    __device__ void flip_range(int lo, int hi) {
+        for (int w = lo >> 5; w <= (hi >> 5); ++w) {
+            uint32_t m = 0xffffffffu;
+            if (w == (lo >> 5)) m &= 0xffffffffu << (lo & 31);
+            if (w == (hi >> 5)) m &= 0xffffffffu >> (31 - (hi & 31));
+            atomicXor(&bits[w * 64 + col], m);
+        }
     }
-}
-
-__device__ __forceinline__ void toggle_exact(const VoxArgs& a, const double* T, double sx, double sy, double ex,
-                                             double ey, uint32_t* bits, int tid, int lo, int hi) {
-    for (int k = lo; k <= hi; ++k)
-        if (t_window(a, T, sx, sy, ex, ey, k)) atomicXor(&bits[(k >> 5) * 64 + tid], 1u << (k & 31));
-}
-
-// Toggles the parity bit of every k whose t lies in the window, for one facet
-// that passed the XY test at this column.
-//
-// With ex == sx and ey == sy the reference's t at height z is, in real
-// arithmetic, (zp - z) / H with H = L + pad and zp = T2 + A / nz, where
-// A = (T0 - sx) nx + (T1 - sy) ny is the reference's own partial sum.  The
-// window is then the interval zp - (1 + teps) H <= z <= zp + teps H.  Every
-// rounding step (z_k, the numerator, the denominator ez - sz, the constant
-// 1 + teps, and this estimate) errs by at most a few units in the last place
-// of the magnitudes involved, so M below, 64 ulp of their sum, bounds how far
-// the rounded decision can move an end of the interval.  Grid points within M
-// of an end (plus one on each side) are decided by the literal expression;
-// those strictly between the two ends are in the window and the rest are not.
-// Pairs where that argument does not hold (non-zero transverse ray component,
-// non-finite or extreme values, an end region wider than 16 points) evaluate
-// every k literally.
-__device__ void window_toggle(const VoxArgs& a, const double* T, double sx, double sy, double ex, double ey,
-                              uint32_t* bits, int tid) {
-    const double nx = T[9], ny = T[10], nz = T[11];
-    const double A = (T[0] - sx) * nx + (T[1] - sy) * ny;
-    const double D0 = (ex - sx) * nx + (ey - sy) * ny;
-    const double H = a.L + a.pad;
-    const double q = A / nz;
-    const double S = fabs(a.oz) + (double)a.nz * a.sp;
-    const double mag = fabs(T[2]) + fabs(q) + 2.0 * (S + fabs(a.L) + fabs(a.pad)) + (2.0 + a.teps) * H;
-    const double M = 64.0 * 0x1p-53 * mag;
-    const bool ok = D0 == 0.0 && fabs(nz) >= 1e-150 && fabs(nz) <= 1e150 && mag <= 1e100 && H > 1e-9 * mag &&
-                    a.teps >= 0.0 && a.teps <= 1e6 && a.sp > 1e-9 * mag && fabs(A) <= 1e250;
-    const int last = a.nz - 1;
-    if (!ok) {
-        toggle_exact(a, T, sx, sy, ex, ey, bits, tid, 0, last);
-        return;
-    }
-    const double zp = T[2] + q;
-    const double zlo = zp - (1.0 + a.teps) * H, zhi = zp + a.teps * H;
-    // grid index range [floor((z - M - oz) / sp) - 1, floor((z + M - oz) / sp) + 1], clamped
-    auto kidx = [&](double z, double off) {
-        const double v = floor((z - a.oz) / a.sp) + off;
-        return (int)fmin(fmax(v, -2.0), (double)a.nz + 1.0);
-    };
-    const int a0 = kidx(zlo - M, -1.0), a1 = kidx(zlo + M, 1.0);
-    const int b0 = kidx(zhi - M, -1.0), b1 = kidx(zhi + M, 1.0);
-    if (a1 - a0 > 16 || b1 - b0 > 16) {
-        toggle_exact(a, T, sx, sy, ex, ey, bits, tid, 0, last);
-        return;
-    }
-    if (b0 <= a1 + 1) {  // the two end regions touch: decide all of them literally
-        toggle_exact(a, T, sx, sy, ex, ey, bits, tid, max(a0, 0), min(b1, last));
-        return;
-    }
-    toggle_exact(a, T, sx, sy, ex, ey, bits, tid, max(a0, 0), min(a1, last));
-    const int m0 = max(a1 + 1, 0), m1 = min(b0 - 1, last);
-    if (m0 <= m1) toggle_range(bits, tid, m0, m1);
-    toggle_exact(a, T, sx, sy, ex, ey, bits, tid, max(b0, 0), min(b1, last));
-}
+};
 
 // The workgroup's wavefronts take the list's batches of 64 facets in turn
 // (batch r * VWAVES + wave in round r), each staging its batch through its own
@@ -177,11 +109,13 @@ __device__ void process_list(const VoxArgs& a, const int32_t* list, int count, d
         }
         __syncthreads();
         if (!valid) continue;
+        const lbm_vox::WinRay g{a.oz, a.sp, a.L, a.pad, a.teps, a.nz};
+        LdsColumnBits col{bits, tid};
         for (int t = 0; t < m; ++t) {
             double T[12];
 #pragma unroll
             for (int c = 0; c < 12; ++c) T[c] = mine[c * 64 + t];
-            if (inside_xy(sx, sy, T)) window_toggle(a, T, sx, sy, ex, ey, bits, tid);
+            if (inside_xy(sx, sy, T)) lbm_vox::window_toggle(g, T, sx, sy, ex, ey, col);
         }
     }
 }

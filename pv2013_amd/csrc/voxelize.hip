@@ -197,6 +197,15 @@ __global__ __launch_bounds__(256) void k_classify(ClsArgs a) {
     a.out[s] = f;
 }
 
+// A pair of timing events, destroyed on every return path.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 struct DevMem {
     std::vector<void*> ptrs;
     ~DevMem() {
@@ -314,9 +323,10 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
     a.L = g->raybox[5] - g->raybox[4];
     a.pad = g->ray_pad; a.off = g->ray_offset; a.teps = g->t_eps;
     a.flags = d_flags;
-    hipEvent_t e0, e1;
-    VCHECK(hipEventCreate(&e0));
-    VCHECK(hipEventCreate(&e1));
+    EventPair ev;
+    VCHECK(hipEventCreate(&ev.e0));
+    VCHECK(hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     VCHECK(hipEventRecord(e0, st));
     const size_t lds_bits = sizeof(uint32_t) * 64 * (size_t)((nz + 31) / 32);
     hipLaunchKernelGGL(k_voxelize, dim3((unsigned)ntiles), dim3(64 * VWAVES), lds_bits, st, a);
@@ -327,8 +337,6 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
     VCHECK(hipStreamSynchronize(st));
     float ms = 0.f;
     VCHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (kernel_ms) *kernel_ms = ms;
     return LBM_OK;
 }
@@ -387,9 +395,10 @@ extern "C" int lbm_classify(lbm_ctx* ctx, const lbm_grid_desc* g, const lbm_plan
     const int64_t bx = (nx + 255) / 256, blocks = bx * ny * (int64_t)nz;
     if (blocks > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_classify: grid too large");
     ClsArgs a{d_in, d_out, nx, ny, nz, nplanes, d_box, (int32_t)bx};
-    hipEvent_t e0, e1;
-    VCHECK(hipEventCreate(&e0));
-    VCHECK(hipEventCreate(&e1));
+    EventPair ev;
+    VCHECK(hipEventCreate(&ev.e0));
+    VCHECK(hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     VCHECK(hipEventRecord(e0, st));
     hipLaunchKernelGGL(k_classify, dim3((unsigned)blocks), dim3(256), 0, st, a);
     hipError_t le = hipGetLastError();
@@ -399,8 +408,6 @@ extern "C" int lbm_classify(lbm_ctx* ctx, const lbm_grid_desc* g, const lbm_plan
     VCHECK(hipStreamSynchronize(st));
     float ms = 0.f;
     VCHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (kernel_ms) *kernel_ms = ms;
     return LBM_OK;
 }

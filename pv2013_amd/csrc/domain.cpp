@@ -667,8 +667,8 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             for (int e = 0; ok && e < q; ++e) {
                 if (e == rest) continue;
                 if (clo[e] >= 0) {
-                    h[6 + 2 * e] = cbase[e];
-                    h[7 + 2 * e] = clo[e] | (chi[e] << 8);
+                    h[6 + e] = cbase[e];
+                    h[6 + q + e / 2] |= (int32_t)((uint32_t)(clo[e] | (chi[e] << 8)) << (16 * (e & 1)));
                     ok = (int64_t)cbase[e] + clo[e] >= 0 && (int64_t)cbase[e] + chi[e] <= D->n;
                 }
                 // a lane without a record has every link

@@ -103,13 +103,16 @@ enum { LBM_LAYOUT_GENERIC = 0, LBM_LAYOUT_FUSED = 1, LBM_LAYOUT_SPLIT = 2 };
 constexpr int LBM_SEG_W = 32;
 
 // Row chunk table entry (fused layout): every node belongs to one chunk, up to
-// 64 consecutive node ids along one row inside one 64-node word.
+// 64 consecutive node ids along one row.
 //   [0] start node, [1] length, [2] records before start, [3] unused,
 //   [4], [5] record mask (bit l: node start + l has a boundary record),
-//   [6 + 2d] base_d, [7 + 2d] lo_d | hi_d << 8: lanes l in [lo_d, hi_d) read
-//   direction d from node base_d + l; the other lanes' source x - c_d is
-//   missing (bounce-back or stale value).
-constexpr int lbm_chunk_w(int q) { return q == 19 ? 48 : 64; }
+//   [6 + d] base_d,
+//   [6 + Q + d / 2] bits 16 (d & 1) .. 16 (d & 1) + 15: lo_d | hi_d << 8:
+//   lanes l in [lo_d, hi_d) read direction d from node base_d + l; the other
+//   lanes' source x - c_d is missing (bounce-back or stale value).
+// 36 ints (144 B) for D3Q19, 48 ints (192 B) for D3Q27.
+constexpr int lbm_chunk_w(int q) { return q == 19 ? 36 : 48; }
+static_assert(6 + 19 + 10 <= 36 && 6 + 27 + 14 <= 48, "chunk row fits");
 
 // Waveform values are tabulated once per chunk of LBM_WCHUNK consecutive
 // iterations: wvals[slot][k] = W_slot(base + k), base = it0 (it_abs) or *iter.

@@ -781,8 +781,8 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     int32_t hb[Q], hl[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-        hb[d] = h[6 + 2 * d];
-        hl[d] = h[7 + 2 * d];
+        hb[d] = h[6 + d];
+        hl[d] = (int32_t)(((uint32_t)h[6 + Q + (d >> 1)] >> (16 * (d & 1))) & 0xffffu);
     }
     T f[Q];
     if ((mlo | mhi) == 0u) {

@@ -187,6 +187,24 @@ def test_batch_matches_individual_runs(P, oracle, ctx):
         assert bits_equal(d.wss(), dom.wss_box(inf["origin"], inf["dims"]))
 
 
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_step_timed_equals_step(P, oracle, ctx, scheme):
+    """lbm_step_timed builds a missing step graph before its first event; the
+    steps it then runs equal lbm_step's bit for bit, from a fresh domain
+    (graph built inside the call) and after an odd number of steps (AA starts
+    its graphs on even iterations)."""
+    dims = (14, 14, 30)
+    cl, bcs = pipe_case(oracle, dims, 5.5, 0.04)
+    doms = [P.Domain(ctx, dims, cl, 0.8, [P.make_bc(**b) for b in bcs], scheme=scheme) for _ in range(2)]
+    for n in (100, 67, 130):
+        ms = doms[0].step_timed(n)
+        doms[1].step(n)
+        assert ms > 0.0
+        assert bits_equal(doms[0].f(), doms[1].f()), n
+    for d in doms:
+        d.close()
+
+
 @pytest.mark.parametrize("storage", [0, 1, 2])
 def test_aa_equals_two_lattice(P, oracle, ctx, storage):
     """The AA scheme (one lattice, alternating gather / local steps) computes

@@ -1,6 +1,7 @@
 #!/bin/bash
-# Builds the reference's inlet Zou-He (Boundary::ApplyBc) from its own sources
-# into oracle/_ref/ (test infrastructure; never shipped).  Needs the reference
+# Builds the reference's inlet Zou-He (Boundary::ApplyBc) and D3Q19 lattice
+# tables (LatticeModel) from its own sources into oracle/_ref/ (test
+# infrastructure; never shipped).  Needs the reference
 # checkout (default /root/reference) and the MPI headers in the image (include.h includes mpi.h;
 # nothing in these files calls MPI, so no MPI library is linked).
 # Compiled files: Utility.cpp LatticeModel.cpp VelocitySet.cpp Node.cpp Boundary.cpp.

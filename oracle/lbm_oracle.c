@@ -11,7 +11,10 @@
  * not just retype the formula):
  *   orc_lattice          LatticeModel.cpp:51-151, 208-334.  Pinned by the closed
  *                        forms sum w = 1, sum w c = 0, sum w c c = I/3, opp(c) = -c
- *                        (tests/test_oracle_lattice.py).
+ *                        (tests/test_oracle_lattice.py) and, for D3Q19, by the
+ *                        reference's own LatticeModel built from its sources:
+ *                        weights bit for bit, directions and opposites exactly
+ *                        (tests/test_oracle_bc.py).
  *   orc_read_stl_ascii   GeometryReader.cpp:33-117.  Pinned by the facet count and
  *                        bbox of PV2013.stl and by the PV2013 voxel fixtures.
  *   orc_bbox / grid      ComplexGeometry.cpp:40-73, 178-207.  Pinned by the

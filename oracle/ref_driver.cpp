@@ -1,12 +1,17 @@
 // Test-only driver around the reference's own sources (compiled from
 // /root/reference by oracle/build_ref.sh into oracle/_ref/).  It exercises
 //   Boundary::ApplyBc (Boundary.cpp:127-148, D3Q19 Zou-He inlet)
+//   LatticeModel (LatticeModel.cpp:51-151): weights, directions, opposite
+//   directions and c_s^2 of the D3Q19 set the reference compiles
 // The reference's voxelizer (ComplexGeometry.cpp) is not built: it includes
 // mpi_call.h, whose member template at :143 redeclares the class template's
 // parameter T, which gcc and clang reject (icc accepted it).
 // Usage:
 //   ref_driver zouhe <in.bin> <out.bin>
 //       reads n then n*19 doubles, applies Boundary::ApplyBc to each node
+//   ref_driver lattice <out.bin>
+//       writes q, then per direction the weight (double) and cx, cy, cz,
+//       opposite (int32), then c_s^2 (double)
 #include <cstdio>
 #include <cstdint>
 #include <fstream>
@@ -35,9 +40,27 @@ static int zouhe(const char* in, const char* out) {
     return 0;
 }
 
+static int lattice(const char* out) {
+    LatticeModel m;
+    std::ofstream o(out, std::ios::binary);
+    const int32_t q = m.Get_nDirections();
+    o.write((const char*)&q, sizeof q);
+    for (int d = 0; d < q; ++d) {
+        const double w = m.GetWeight(d);
+        const std::vector<int> c = m.GetDirection(d);
+        const int32_t rec[4] = {c[0], c[1], c[2], m.OppositeDirection(d)};
+        o.write((const char*)&w, sizeof w);
+        o.write((const char*)rec, sizeof rec);
+    }
+    const double cs2 = m.Get_c_square(0);
+    o.write((const char*)&cs2, sizeof cs2);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     int rc = 2;
     if (argc == 4 && std::string(argv[1]) == "zouhe") rc = zouhe(argv[2], argv[3]);
-    else fprintf(stderr, "usage: ref_driver zouhe <in> <out>\n");
+    else if (argc == 3 && std::string(argv[1]) == "lattice") rc = lattice(argv[2]);
+    else fprintf(stderr, "usage: ref_driver zouhe <in> <out> | ref_driver lattice <out>\n");
     return rc;
 }

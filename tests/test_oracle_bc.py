@@ -1,5 +1,6 @@
-"""Zou-He, bounce-back and collision of the oracle checked against the
-reference's own Boundary::ApplyBc (built from its sources) and moment identities."""
+"""Zou-He, bounce-back, collision and the lattice tables of the oracle checked
+against the reference's own Boundary::ApplyBc and LatticeModel (built from its
+sources) and moment identities."""
 import os
 import subprocess
 
@@ -36,6 +37,24 @@ def test_inlet_matches_reference_build(oracle, tmp_path):
     bc = oracle.make_bc(0, 1, -1, u=(0.0, -0.1, 0.0))
     mine = np.array([oracle.zouhe_node(19, bc, (0.0, -0.1, 0.0), row) for row in f])
     assert np.array_equal(mine.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference build not present (needs /root/reference)")
+def test_lattice_matches_reference_build(oracle, tmp_path):
+    """The oracle's D3Q19 tables against the reference's own LatticeModel
+    (LatticeModel.cpp:51-151): weights bit for bit, directions and the
+    opposite-direction map exactly, and c_s^2 = 1/3."""
+    dst = tmp_path / "lattice.bin"
+    subprocess.check_call([REF, "lattice", str(dst)])
+    raw = dst.read_bytes()
+    q = int(np.frombuffer(raw[:4], np.int32)[0])
+    rec = np.frombuffer(raw[4:4 + q * 24], dtype=[("w", "<f8"), ("c", "<i4", 3), ("opp", "<i4")])
+    cs2 = np.frombuffer(raw[4 + q * 24:], np.float64)[0]
+    c, w, opp = oracle.lattice(19)
+    assert q == 19
+    assert np.array_equal(rec["w"].view(np.uint64), np.asarray(w, np.float64).view(np.uint64))
+    assert np.array_equal(rec["c"], c) and np.array_equal(rec["opp"], opp)
+    assert cs2 == 1.0 / 3.0
 
 
 def moments(q, f):

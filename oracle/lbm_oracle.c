@@ -16,7 +16,9 @@
  *                        weights bit for bit, directions and opposites exactly
  *                        (tests/test_oracle_bc.py).
  *   orc_read_stl_ascii   GeometryReader.cpp:33-117.  Pinned by the facet count and
- *                        bbox of PV2013.stl and by the PV2013 voxel fixtures.
+ *                        bbox of PV2013.stl, by the PV2013 voxel fixtures, and bit
+ *                        for bit by the reference's own Reader built from its
+ *                        sources (tests/test_capi_cpu.py).
  *   orc_bbox / grid      ComplexGeometry.cpp:40-73, 178-207.  Pinned by the
  *                        PV2013 fixtures (every fixture point maps to a grid index).
  *   orc_voxelize         ComplexGeometry.cpp:264-357, 431-506.  Pinned by the

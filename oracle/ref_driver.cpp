@@ -3,6 +3,7 @@
 //   Boundary::ApplyBc (Boundary.cpp:127-148, D3Q19 Zou-He inlet)
 //   LatticeModel (LatticeModel.cpp:51-151): weights, directions, opposite
 //   directions and c_s^2 of the D3Q19 set the reference compiles
+//   Reader::readSTLASCII / readSTLBinary (GeometryReader.cpp:33-176)
 // The reference's voxelizer (ComplexGeometry.cpp) is not built: it includes
 // mpi_call.h, whose member template at :143 redeclares the class template's
 // parameter T, which gcc and clang reject (icc accepted it).
@@ -12,6 +13,8 @@
 //   ref_driver lattice <out.bin>
 //       writes q, then per direction the weight (double) and cx, cy, cz,
 //       opposite (int32), then c_s^2 (double)
+//   ref_driver stl ascii|binary <in.stl> <out.bin>
+//       writes n, then per facet v1, v2, v3 and the normal (12 doubles)
 #include <cstdio>
 #include <cstdint>
 #include <fstream>
@@ -20,6 +23,7 @@
 #include <string>
 #include "LatticeModel.h"
 #include "Boundary.h"
+#include "GeometryReader.h"
 
 static int zouhe(const char* in, const char* out) {
     std::ifstream i(in, std::ios::binary);
@@ -57,10 +61,24 @@ static int lattice(const char* out) {
     return 0;
 }
 
+static int stl(const std::string& kind, const char* in, const char* out) {
+    Reader r;
+    const std::vector<Triangle> t = kind == "ascii" ? r.readSTLASCII(in) : r.readSTLBinary(in);
+    std::ofstream o(out, std::ios::binary);
+    const int64_t n = (int64_t)t.size();
+    o.write((const char*)&n, sizeof n);
+    for (const Triangle& x : t) {
+        const double v[12] = {x.x1, x.y1, x.z1, x.x2, x.y2, x.z2, x.x3, x.y3, x.z3, x.nx, x.ny, x.nz};
+        o.write((const char*)v, sizeof v);
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     int rc = 2;
     if (argc == 4 && std::string(argv[1]) == "zouhe") rc = zouhe(argv[2], argv[3]);
     else if (argc == 3 && std::string(argv[1]) == "lattice") rc = lattice(argv[2]);
-    else fprintf(stderr, "usage: ref_driver zouhe <in> <out> | ref_driver lattice <out>\n");
+    else if (argc == 5 && std::string(argv[1]) == "stl") rc = stl(argv[2], argv[3], argv[4]);
+    else fprintf(stderr, "usage: ref_driver zouhe <in> <out> | lattice <out> | stl ascii|binary <in> <out>\n");
     return rc;
 }

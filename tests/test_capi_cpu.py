@@ -74,6 +74,55 @@ def test_stl_binary_renormalises_in_float(P, tmp_path):
     assert np.array_equal(got[:, 9:], (n / norm[:, None]).astype(np.float32).astype(np.float64))
 
 
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+
+def ref_stl(kind, path, out):
+    import subprocess
+    subprocess.check_call([REF, "stl", kind, path, out])
+    raw = open(out, "rb").read()
+    n = int(np.frombuffer(raw[:8], np.int64)[0])
+    return np.frombuffer(raw[8:], np.float64).reshape(n, 12)
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference build not present (needs /root/reference)")
+def test_stl_readers_match_reference_build(P, oracle, tmp_path):
+    """The library's and the oracle's STL readers against the reference's own
+    Reader::readSTLASCII / readSTLBinary (GeometryReader.cpp:33-176), built from
+    its sources: the PV2013 facets written back as ASCII, an ASCII file in
+    awkward number formats, and a binary file whose normals are not unit
+    length (the reference renormalises them in float)."""
+    bits = lambda a: np.ascontiguousarray(a, np.float64).view(np.uint64)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "pv2013.npz"))
+    a1 = str(tmp_path / "pv.stl")
+    write_ascii_stl(a1, z["tri"])
+    a2 = str(tmp_path / "odd.stl")
+    with open(a2, "w") as fh:
+        fh.write("solid odd\n")
+        for nrm, verts in [("0 0 1", ["1e-3 +2.5 -0", "1E+02 3 4.", "0.1 0.2 0.30000000000000004"]),
+                           ("-0.0 1.0e0 0", ["123456789.123456789 -1e-300 7", ".5 -.25 1e2", "3 2 1"]),
+                           ("0.6 0.8 0", ["2.2250738585072014e-308 1 1", "9007199254740993 0 0", "1 2 3"])]:
+            fh.write("  facet normal %s\n    outer loop\n" % nrm)
+            for v in verts:
+                fh.write("      vertex %s\n" % v)
+            fh.write("    endloop\n  endfacet\n")
+        fh.write("endsolid odd\n")
+    rng = np.random.default_rng(3)
+    b = str(tmp_path / "b.stl")
+    with open(b, "wb") as fh:
+        fh.write(b"\0" * 80 + struct.pack("<I", 200))
+        for i in range(200):
+            n = (rng.standard_normal(3) * 10 ** rng.uniform(-3, 3)).astype(np.float32)
+            v = (rng.standard_normal(9) * 100).astype(np.float32)
+            fh.write(n.tobytes() + v.tobytes() + b"\0\0")
+    for kind, path in (("ascii", a1), ("ascii", a2), ("binary", b)):
+        ref = ref_stl(kind, path, str(tmp_path / "ref.bin"))
+        assert len(ref) > 0
+        assert np.array_equal(bits(P.read_stl(path)), bits(ref)), path
+        if kind == "ascii":
+            assert np.array_equal(bits(oracle.read_stl_ascii(path)), bits(ref)), path
+
+
 def test_calls_fail_cleanly_without_context(P):
     L = P.lib()
     h = C.c_void_p()

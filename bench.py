@@ -30,7 +30,7 @@ SEED = 2013
 PERIOD = 4000.0
 
 
-def run(P, ctx, case, storage, steps, warmup, scheme=0, options=0, block=256):
+def run(P, ctx, case, storage, steps, warmup, scheme=0, options=0, block=0):
     from pv2013_amd.cases import make_domain
     dom = make_domain(ctx, case, storage=storage, scheme=scheme, options=options, block=block)
     st = dom.stats()
@@ -119,8 +119,9 @@ def main():
                     help="skip the FP64 / FP16 comparison runs and the accuracy protocol")
     ap.add_argument("--scheme", default="aa", choices=["two", "aa"],
                     help="streaming: AA (one lattice, default) or two-lattice pull; identical results")
-    ap.add_argument("--block", type=int, default=256, choices=[64, 128, 256, 512],
-                    help="threads per block of the step kernels")
+    ap.add_argument("--block", type=int, default=0, choices=[0, 64, 128, 256, 512],
+                    help="threads per block of the step kernels (0: the library default, 512 for FP32 "
+                         "storage and 256 otherwise)")
     ap.add_argument("--layout", default="fused", choices=["fused", "split", "generic"],
                     help="step kernel: row chunks in one launch (default), records + row segments "
                          "in two, or the generic per-node kernel")
@@ -151,7 +152,7 @@ def main():
              "wall_s_timed": r["wall_s"], "mass_after": r["mass"], "facets": case["ntri"],
              "voxelize_ms": case["vox_ms"], "classify_ms": case["cls_ms"], "scheme": args.scheme,
              "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3),
-             "layout": args.layout, "block": args.block, "boundary_records": r["stats"]["boundary_records"],
+             "layout": args.layout, "block": r["stats"]["block"], "boundary_records": r["stats"]["boundary_records"],
              "row_chunks" if args.layout == "fused" else "segments": r["stats"]["segments"],
              "lane_use": ((n if args.layout == "fused" else n - r["stats"]["boundary_records"])
                           / max(1, 64 * r["stats"]["segments"])),

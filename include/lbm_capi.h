@@ -137,7 +137,8 @@ typedef struct {
     int32_t storage;  /* LBM_STORE_* */
     int32_t scheme;   /* LBM_SCHEME_* */
     int32_t quirks;   /* 1: the reference's step semantics; 0: clean bounce-back */
-    int32_t block;    /* step-kernel threads per block: 64, 128, 256 or 512; 0 = 256 */
+    int32_t block;    /* step-kernel threads per block: 64, 128, 256 or 512; 0 = 512 for
+                         FP32 storage, 256 otherwise (the faster on the benchmark case) */
     double tau;
     const uint8_t* flags;    /* host, dims product */
     const lbm_bc_desc* bcs;  /* bcs[b] drives flag 3+b */
@@ -192,7 +193,7 @@ typedef struct {
     int32_t fast_axis;             /* node-order fastest axis */
     int32_t scheme;
     int32_t layout;                /* 0 generic kernel, 1 row chunks (one launch), 2 records + segments (two) */
-    int32_t pad_;
+    int32_t block;                 /* step-kernel threads per block in use */
 } lbm_domain_stats;
 int lbm_domain_get_stats(const lbm_domain* dom, lbm_domain_stats* out);
 

@@ -71,7 +71,7 @@ class DomainStats(C.Structure):
     _fields_ = [("nodes", C.c_int64), ("boundary_records", C.c_int64), ("wall_nodes", C.c_int64),
                 ("bc_nodes", C.c_int64), ("waves", C.c_int64), ("regular_waves", C.c_int64),
                 ("segments", C.c_int64), ("device_bytes", C.c_int64), ("fast_axis", C.c_int32),
-                ("scheme", C.c_int32), ("layout", C.c_int32), ("pad_", C.c_int32)]
+                ("scheme", C.c_int32), ("layout", C.c_int32), ("block", C.c_int32)]
 
 
 class TuneResult(C.Structure):

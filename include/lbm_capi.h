@@ -137,8 +137,8 @@ typedef struct {
     int32_t storage;  /* LBM_STORE_* */
     int32_t scheme;   /* LBM_SCHEME_* */
     int32_t quirks;   /* 1: the reference's step semantics; 0: clean bounce-back */
-    int32_t block;    /* step-kernel threads per block: 64, 128, 256 or 512; 0 = 512 for
-                         FP32 storage, 256 otherwise (the faster on the benchmark case) */
+    int32_t block;    /* step-kernel threads per block: 64, 128, 256 or 512; 0 = 64 for
+                         FP16 storage, 512 otherwise (the fastest on the benchmark case) */
     double tau;
     const uint8_t* flags;    /* host, dims product */
     const lbm_bc_desc* bcs;  /* bcs[b] drives flag 3+b */

@@ -383,7 +383,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     D->scheme = d.scheme;
     D->quirks = d.quirks;
     D->tau = d.tau;
-    D->block = d.block > 0 ? d.block : (d.storage == LBM_STORE_F32 ? 512 : 256);
+    D->block = d.block > 0 ? d.block : (d.storage == LBM_STORE_F16S ? 64 : 512);
     if (D->block != 64 && D->block != 128 && D->block != 256 && D->block != 512) {
         delete D;
         return lbm_ctx_fail(ctx, LBM_EINVAL, "block must be 64, 128, 256 or 512 threads");

@@ -335,7 +335,7 @@ def test_autotune_picks_a_valid_layout_and_caches_it(P, oracle, ctx, tmp_path):
     assert bits_equal(dom.f(), ref.f())
 
 
-@pytest.mark.parametrize("block", [64, 128, 512])
+@pytest.mark.parametrize("block", [64, 128, 256, 512])
 def test_block_sizes_give_identical_bits(P, oracle, ctx, block):
     dims = (10, 11, 12)
     cl, bcs = six_face_box(oracle, dims)

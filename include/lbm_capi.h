@@ -249,8 +249,8 @@ typedef struct {
 
 /* Times each candidate layout of `desc` (streaming scheme two-lattice / AA,
  * node-order fast axis x / y / z, kernel: row chunks in one launch / records
- * and segments in two / generic; then 128 / 256 / 512 threads per block for
- * the winner) for `steps` steps
+ * and segments in two / generic, at the default block size for the storage;
+ * then the other block sizes of 64 / 128 / 256 / 512 for the winner) for `steps` steps
  * after a warm-up and returns the fastest.  When cache_path is not NULL the
  * result is looked up there first (key: lattice, storage, dims, flag hash) and
  * appended after tuning.  No reference counterpart. */

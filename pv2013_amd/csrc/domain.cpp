@@ -191,6 +191,10 @@ struct lbm_domain {
 
 static size_t store_bytes(int storage) { return storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2); }
 
+// Step-kernel threads per block when the description leaves it 0: the fastest
+// on the benchmark case (FP16 storage: 64; FP32 and FP64: 512).
+static int default_block(int storage) { return storage == LBM_STORE_F16S ? 64 : 512; }
+
 static int choose_fast_axis(const uint8_t* flags, const int* dims) {
     // fewest runs per fluid site = longest runs
     int64_t runs[3] = {0, 0, 0};
@@ -383,7 +387,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     D->scheme = d.scheme;
     D->quirks = d.quirks;
     D->tau = d.tau;
-    D->block = d.block > 0 ? d.block : (d.storage == LBM_STORE_F16S ? 64 : 512);
+    D->block = d.block > 0 ? d.block : default_block(d.storage);
     if (D->block != 64 && D->block != 128 && D->block != 256 && D->block != 512) {
         delete D;
         return lbm_ctx_fail(ctx, LBM_EINVAL, "block must be 64, 128, 256 or 512 threads");
@@ -1423,17 +1427,18 @@ extern "C" int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps
         }
         return LBM_OK;
     };
-    // stage 1: scheme x node order x kernel layout at 256 threads per block
+    // stage 1: scheme x node order x kernel layout at the storage's default block size
     static const int kern_opt[3] = {0, LBM_OPT_SPLIT_LAUNCH, LBM_OPT_GENERIC_KERNEL};
     const int keep = desc->options & ~(LBM_OPT_NO_ROW_FASTPATH | LBM_OPT_GENERIC_KERNEL | LBM_OPT_SPLIT_LAUNCH);
     for (int sch = 0; sch < 2; ++sch)
         for (int ax = 0; ax < 3; ++ax)
             for (int kern = 0; kern < 3; ++kern)
-                if (int rc = trial(sch, ax, keep | kern_opt[kern], 256)) return rc;
-    // stage 2: block size for the winner
+                if (int rc = trial(sch, ax, keep | kern_opt[kern], default_block(desc->storage))) return rc;
+    // stage 2: the other block sizes for the winner
     const int bs = out->scheme, ba = out->fast_axis, bo = out->options;
-    for (int blk : {128, 512})
-        if (int rc = trial(bs, ba, bo, blk)) return rc;
+    for (int blk : {64, 128, 256, 512})
+        if (blk != default_block(desc->storage))
+            if (int rc = trial(bs, ba, bo, blk)) return rc;
     if (cache_path) {
         if (FILE* fp = std::fopen(cache_path, "a")) {
             std::fprintf(fp, "%llx %d %d %d %d %.6f\n", (unsigned long long)key, out->scheme, out->fast_axis,

@@ -74,6 +74,23 @@ def test_stl_binary_renormalises_in_float(P, tmp_path):
     assert np.array_equal(got[:, 9:], (n / norm[:, None]).astype(np.float32).astype(np.float64))
 
 
+def test_header_compiles_as_c99(tmp_path):
+    """include/lbm_capi.h is plain C: a C99 translation unit that takes the
+    address of every declared function compiles with -pedantic -Werror (what a
+    cgo or other C-FFI binding sees)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        pytest.skip("no C compiler")
+    names = declared()
+    src = tmp_path / "use_capi.c"
+    src.write_text('#include "lbm_capi.h"\ntypedef void (*fn_t)(void);\nfn_t all[] = {\n' +
+                   "".join("    (fn_t)%s,\n" % n for n in names) + "};\nint main(void) { return all[0] == 0; }\n")
+    subprocess.check_call([cc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I",
+                           os.path.join(ROOT, "include"), "-c", str(src), "-o", str(tmp_path / "use_capi.o")])
+
+
 REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 
 

@@ -30,6 +30,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "lbm_internal.h"
@@ -261,42 +262,86 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
     const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
     const int64_t ntiles = (int64_t)ntx * nty;
     if (ntiles > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: more than 2^31 column tiles");
-    // bin facets into column tiles
-    std::vector<int32_t> keep, global;
+    // Bin facets into column tiles.  Up to 16 host threads take contiguous facet
+    // ranges with per-thread tile counts; the counts then become per-thread
+    // write offsets, so every tile's list holds its facets in facet order, as a
+    // single pass would (the flags do not depend on the order either way).
+    struct Part {
+        std::vector<int32_t> keep, global, cnt;
+        std::vector<std::array<int32_t, 4>> rng;
+    };
+    const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    int nthr = (int)std::min<int64_t>(std::min<int64_t>(16, hw), ntri / 16384 + 1);
+    nthr = (int)std::max<int64_t>(1, std::min<int64_t>(nthr, ((int64_t)1 << 26) / ntiles));
+    std::vector<Part> parts(nthr);
+    bool oom = false;
+    auto run = [&](auto&& work) {
+        std::vector<std::thread> th;
+        std::vector<char> failed(nthr, 0);
+        auto guarded = [&](int p) {
+            try {
+                work(p);
+            } catch (...) {
+                failed[p] = 1;
+            }
+        };
+        for (int p = 1; p < nthr; ++p) th.emplace_back(guarded, p);
+        guarded(0);
+        for (auto& t : th) t.join();
+        for (char f : failed) oom = oom || f;
+    };
+    const double slack = margin + g->ray_offset;
+    run([&](int p) {
+        Part& P = parts[p];
+        P.cnt.assign((size_t)ntiles, 0);
+        for (int64_t t = ntri * p / nthr; t < ntri * (p + 1) / nthr; ++t) {
+            const double* T = tri + 12 * t;
+            if (T[11] == 0.0) continue;  // t = +-inf or NaN: never counted
+            if (thin_xy(T)) { P.global.push_back((int32_t)t); continue; }
+            const double xlo = std::min({T[0], T[3], T[6]}) - slack, xhi = std::max({T[0], T[3], T[6]}) + slack;
+            const double ylo = std::min({T[1], T[4], T[7]}) - slack, yhi = std::max({T[1], T[4], T[7]}) + slack;
+            long i0 = (long)std::floor((xlo - ox) / sp) - 1, i1 = (long)std::ceil((xhi - ox) / sp) + 1;
+            long j0 = (long)std::floor((ylo - oy) / sp) - 1, j1 = (long)std::ceil((yhi - oy) / sp) + 1;
+            i0 = std::max(i0, 0L); j0 = std::max(j0, 0L);
+            i1 = std::min(i1, (long)nx - 1); j1 = std::min(j1, (long)ny - 1);
+            if (i0 > i1 || j0 > j1) continue;
+            std::array<int32_t, 4> r = {(int32_t)(i0 / TX), (int32_t)(i1 / TX), (int32_t)(j0 / TY),
+                                        (int32_t)(j1 / TY)};
+            P.keep.push_back((int32_t)t);
+            P.rng.push_back(r);
+            for (int ty = r[2]; ty <= r[3]; ++ty)
+                for (int tx = r[0]; tx <= r[1]; ++tx) P.cnt[(int64_t)ty * ntx + tx]++;
+        }
+    });
+    if (oom) return lbm_ctx_fail(ctx, LBM_ENOMEM, "lbm_voxelize: out of host memory while binning");
+    // tile offsets, then each thread's write offset inside every tile
     std::vector<int32_t> cnt(ntiles + 1, 0);
-    std::vector<std::array<int32_t, 4>> rng;
-    for (int64_t t = 0; t < ntri; ++t) {
-        const double* T = tri + 12 * t;
-        if (T[11] == 0.0) continue;  // t = +-inf or NaN: never counted
-        if (thin_xy(T)) { global.push_back((int32_t)t); continue; }
-        const double slack = margin + g->ray_offset;
-        const double xlo = std::min({T[0], T[3], T[6]}) - slack, xhi = std::max({T[0], T[3], T[6]}) + slack;
-        const double ylo = std::min({T[1], T[4], T[7]}) - slack, yhi = std::max({T[1], T[4], T[7]}) + slack;
-        long i0 = (long)std::floor((xlo - ox) / sp) - 1, i1 = (long)std::ceil((xhi - ox) / sp) + 1;
-        long j0 = (long)std::floor((ylo - oy) / sp) - 1, j1 = (long)std::ceil((yhi - oy) / sp) + 1;
-        i0 = std::max(i0, 0L); j0 = std::max(j0, 0L);
-        i1 = std::min(i1, (long)nx - 1); j1 = std::min(j1, (long)ny - 1);
-        if (i0 > i1 || j0 > j1) continue;
-        std::array<int32_t, 4> r = {(int32_t)(i0 / TX), (int32_t)(i1 / TX), (int32_t)(j0 / TY),
-                                    (int32_t)(j1 / TY)};
-        keep.push_back((int32_t)t);
-        rng.push_back(r);
-        for (int ty = r[2]; ty <= r[3]; ++ty)
-            for (int tx = r[0]; tx <= r[1]; ++tx) cnt[(int64_t)ty * ntx + tx + 1]++;
+    int64_t total = 0;
+    for (int64_t i = 0; i < ntiles; ++i) {
+        cnt[i] = (int32_t)std::min<int64_t>(total, INT32_MAX);
+        for (int p = 0; p < nthr; ++p) {
+            const int32_t c = parts[p].cnt[i];
+            parts[p].cnt[i] = (int32_t)std::min<int64_t>(total, INT32_MAX);
+            total += c;
+        }
     }
-    for (int64_t i = 0; i < ntiles; ++i) cnt[i + 1] += cnt[i];
-    if (cnt[ntiles] > INT32_MAX / 2) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: bin lists too long");
-    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1), lists(std::max<int64_t>(cnt[ntiles], 1));
+    if (total > INT32_MAX / 2) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: bin lists too long");
+    cnt[ntiles] = (int32_t)total;
+    std::vector<int32_t> lists(std::max<int64_t>(total, 1)), global;
+    for (const Part& P : parts) global.insert(global.end(), P.global.begin(), P.global.end());
     // longest lists first, so the slowest workgroups start first
     std::vector<int32_t> order(ntiles);
     for (int64_t i = 0; i < ntiles; ++i) order[i] = (int32_t)i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t x, int32_t y) { return cnt[x + 1] - cnt[x] > cnt[y + 1] - cnt[y]; });
-    for (size_t u = 0; u < keep.size(); ++u) {
-        const auto& r = rng[u];
-        for (int ty = r[2]; ty <= r[3]; ++ty)
-            for (int tx = r[0]; tx <= r[1]; ++tx) lists[fill[(int64_t)ty * ntx + tx]++] = keep[u];
-    }
+    run([&](int p) {
+        Part& P = parts[p];
+        for (size_t u = 0; u < P.keep.size(); ++u) {
+            const auto& r = P.rng[u];
+            for (int ty = r[2]; ty <= r[3]; ++ty)
+                for (int tx = r[0]; tx <= r[1]; ++tx) lists[P.cnt[(int64_t)ty * ntx + tx]++] = P.keep[u];
+        }
+    });
     DevMem m;
     double* d_tri;
     int32_t *d_off, *d_list, *d_glob, *d_order;

@@ -158,6 +158,10 @@ struct lbm_domain {
     double *fpre = nullptr, *rho = nullptr, *u = nullptr, *wss = nullptr, *red = nullptr, *red_scratch = nullptr;
     int cur = 0;
     int64_t iter = 0;
+    // AA: iteration at which the lattice was last (re)initialised in the
+    // gather-ready layout; the step that starts at iteration it is a gather
+    // when it - aa_base is even, a local step otherwise
+    int64_t aa_base = 0;
     int state = 0;            // 0: f = w (initial), 1: fpre holds f, 2: recompute from g[cur ^ 1]
     int64_t pre_iter = -1;    // iteration for which fpre/rho/u are valid
     int32_t off[27];
@@ -309,11 +313,16 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     for (int d = 0; d < 27; ++d) a.off[d] = D->off[d];
 }
 
-// Streaming scheme of the step that starts at iteration `it`.
-static int scheme_for(const lbm_domain* D, int64_t it) {
+// Streaming scheme of the step `rel` steps after the last (re)initialisation
+// of the lattice (AA: gather on even, local on odd).
+static int scheme_rel(const lbm_domain* D, int64_t rel) {
     if (D->scheme == LBM_SCHEME_TWO_LATTICE) return 0;
-    return (it & 1) == 0 ? 1 : 2;  // AA: gather on even iterations, local on odd
+    return (rel & 1) == 0 ? 1 : 2;
 }
+// Streaming scheme of the step that starts at iteration `it`.
+static int scheme_for(const lbm_domain* D, int64_t it) { return scheme_rel(D, it - D->aa_base); }
+// AA: the next step is a local step (graphs start with a gather).
+static bool aa_odd(const lbm_domain* D) { return D->scheme == LBM_SCHEME_AA && ((D->iter - D->aa_base) & 1); }
 
 // wk: the launch's row of the waveform table (its iteration minus the base of
 // the chunk the table holds).
@@ -870,8 +879,8 @@ extern "C" int lbm_set_waveform(lbm_domain* D, int slot, int kind, double period
     return LBM_OK;
 }
 
-// Captures GRAPH_STEPS steps starting from lattice `c` (two-lattice) or from
-// an even iteration (AA) into a graph.  Kernel arguments are fixed per node
+// Captures GRAPH_STEPS steps starting from lattice `c` (two-lattice) or with
+// a gather step (AA) into a graph.  Kernel arguments are fixed per node
 // (the lattice parity / AA step kind alternates inside the graph); the
 // iteration counter lives in device memory, so replays need no updates.
 // Nothing captured uses a memset node.
@@ -885,7 +894,7 @@ static int build_graph(lbm_domain* D, int c) {
         hipSuccess)
         rc = LBM_EIO;
     for (int i = 0; i < lbm_domain::GRAPH_STEPS && rc == LBM_OK; ++i)
-        rc = launch_step(D, LBM_MODE_STEP, scheme_for(D, i), D->g[(c + i) & 1], D->g[(c + i + 1) & 1], i);
+        rc = launch_step(D, LBM_MODE_STEP, scheme_rel(D, i), D->g[(c + i) & 1], D->g[(c + i + 1) & 1], i);
     if (rc == LBM_OK && launch_advance_iter(D->d_iter, lbm_domain::GRAPH_STEPS, ctx->stream) != hipSuccess)
         rc = LBM_EIO;
     hipGraph_t g = nullptr;
@@ -932,7 +941,7 @@ static int prepare_graph(lbm_domain* D, int64_t nsteps) {
     if (nsteps <= 0 || (D->options & LBM_OPT_NO_GRAPH)) return LBM_OK;
     const bool aa = D->scheme == LBM_SCHEME_AA;
     int64_t body = aa ? nsteps - 1 : nsteps;
-    if (aa && (D->iter & 1) && body > 0) body--;
+    if (aa_odd(D) && body > 0) body--;
     const int c = aa ? 0 : D->cur;
     if (body >= lbm_domain::GRAPH_STEPS && !D->exec[c]) return build_graph(D, c);
     return LBM_OK;
@@ -947,7 +956,7 @@ static int step_n(lbm_domain* D, int64_t nsteps) {
     int64_t body = aa ? nsteps - 1 : nsteps;
     const int64_t G = lbm_domain::GRAPH_STEPS;
     if (!(D->options & LBM_OPT_NO_GRAPH)) {
-        if (aa && (D->iter & 1) && body > 0) {  // AA graphs start on an even iteration
+        if (aa_odd(D) && body > 0) {  // AA graphs start with a gather step
             int rc = eager_step(D, LBM_MODE_STEP);
             if (rc) return rc;
             body--;
@@ -961,7 +970,7 @@ static int step_n(lbm_domain* D, int64_t nsteps) {
             }
             CHECK(hipGraphLaunch(D->exec[c], ctx->stream));
             D->wbase = -1;  // the replay re-tabulated wvals for its own chunk
-            D->iter += G;  // G is even: the lattice parity is unchanged
+            D->iter += G;  // G is even: the lattice parity and the AA phase are unchanged
             body -= G;
         }
     }
@@ -1151,6 +1160,9 @@ extern "C" int lbm_set_f(lbm_domain* D, const double* f) {
     if ((rc = upload_stale(D, stale))) return rc;
     if ((rc = init_from(D, D->fpre))) return rc;
     CHECK(hipStreamSynchronize(ctx->stream));
+    // init_from wrote the gather-ready layout, whatever the iteration: the AA
+    // step sequence restarts with a gather at this iteration
+    D->aa_base = D->iter;
     D->state = 1;
     D->pre_iter = -1;
     return LBM_OK;
@@ -1289,12 +1301,21 @@ extern "C" int lbm_batch_create(lbm_ctx* ctx, const lbm_domain_desc* cases, int 
     int nbc = 0, nwave = 0;
     std::vector<int> bc_base(ncases), wave_base(ncases);
     std::vector<int64_t> base(ncases);
+    // Periodic wrap would happen at the packed box's faces, linking one case's
+    // face to another case (along the stack axis) or to solid padding: the
+    // cases would no longer be independent.
+    const int periodic_bits = LBM_OPT_PERIODIC_X | LBM_OPT_PERIODIC_Y | LBM_OPT_PERIODIC_Z;
     for (int c = 0; c < ncases; ++c) {
         const lbm_domain_desc& d = cases[c];
         if (!d.flags || d.q != c0.q || d.storage != c0.storage || d.scheme != c0.scheme ||
             d.quirks != c0.quirks || d.tau != c0.tau || d.nbc < 0 || (d.nbc > 0 && !d.bcs))
             return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_batch_create: case " + std::to_string(c) +
                                                      " differs in q/storage/scheme/quirks/tau or is empty");
+        if (d.options != c0.options || d.fast_axis != c0.fast_axis || d.block != c0.block)
+            return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_batch_create: case " + std::to_string(c) +
+                                                     " differs in options/fast_axis/block");
+        if (d.options & periodic_bits)
+            return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_batch_create: periodic axes are not supported in a batch");
         for (int a = 0; a < 3; ++a)
             if (d.dims[a] <= 0) return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_batch_create: bad dims");
         base[c] = dims[stack_axis] + (c > 0 ? 1 : 0);

@@ -1,8 +1,6 @@
 #include "ConfigReader.h"
 
-#include <cstring>
 #include <fstream>
-#include <iostream>
 #include <stdexcept>
 
 ConfigReader::ConfigReader(const std::string& file) : configFile(file) {}
@@ -52,35 +50,4 @@ bool ConfigReader::readConfigStrings(std::multimap<std::string, std::string>& en
     while (std::getline(file, line))
         if (split_line(line, key, value)) entries.emplace(key, value);
     return true;
-}
-
-bool ConfigReader::readEquations(std::vector<std::pair<std::string, std::string>>& equations) {
-    std::ifstream file(configFile);
-    if (!file.is_open()) {
-        err_ = "Error opening file: " + configFile;
-        return false;
-    }
-    std::string line, equation, conditions;
-    while (std::getline(file, line)) {
-        if (line.empty() || line[0] == '#') continue;
-        const size_t eq = line.find("Equation");
-        const size_t co = line.find("Conditions:");
-        if (eq != std::string::npos) {
-            if (!equation.empty() && !conditions.empty()) {
-                equations.emplace_back(conditions, equation);
-                equation.clear();
-                conditions.clear();
-            }
-        } else if (co != std::string::npos) {
-            conditions = line.substr(co + std::strlen("Conditions:"));
-        } else {
-            equation += line + '\n';
-        }
-    }
-    if (!equation.empty() && !conditions.empty()) equations.emplace_back(conditions, equation);
-    return true;
-}
-
-void ConfigReader::printEquations(const std::vector<std::pair<std::string, std::string>>& equations) const {
-    for (const auto& e : equations) std::cout << "Equation Section: " << e.first << "\n" << e.second << "\n\n";
 }

@@ -1,12 +1,9 @@
-// Key/value configuration (ConfigReader.h:9-12 interface; grammar of the
-// commented-out ConfigReader.cpp:13-96: "key value" lines, '#' comments,
-// "Equation ... / Conditions:" blocks).
+// Key/value configuration (ConfigReader.h:9-10 interface, ConfigReader.cpp:13-51
+// grammar): "key value" lines, '#' comments.
 #pragma once
 
 #include <map>
 #include <string>
-#include <utility>
-#include <vector>
 
 class ConfigReader {
 public:
@@ -16,8 +13,6 @@ public:
     bool readConfigData(std::map<std::string, double>& constants);
     // Every "key value..." line; repeated keys keep all their values in order.
     bool readConfigStrings(std::multimap<std::string, std::string>& entries);
-    bool readEquations(std::vector<std::pair<std::string, std::string>>& equations);
-    void printEquations(const std::vector<std::pair<std::string, std::string>>& equations) const;
     const std::string& error() const { return err_; }
 
 private:

@@ -509,7 +509,6 @@ struct orc_sim {
     int nbc;
     orc_bc* bcs;
     wave_t wave[MAXWAVE];
-    int pv_first;    /* Simulation.cpp:876 static isFirstCycle */
 };
 
 int64_t orc_sim_nodes(const orc_sim* s) { return s->n; }
@@ -537,7 +536,6 @@ orc_sim* orc_sim_create_periodic(int q, const int32_t dims[3], const uint8_t* fl
     s->nbc = nbc;
     s->bcs = (orc_bc*)malloc(sizeof(orc_bc) * (nbc > 0 ? nbc : 1));
     if (nbc) memcpy(s->bcs, bcs, sizeof(orc_bc) * nbc);
-    s->pv_first = 1;
     s->node = (int64_t*)malloc(sizeof(int64_t) * s->nsites);
     int64_t n = 0;
     /* Subdomain.cpp:638-651: node list in i-j-k order, i outermost */
@@ -633,13 +631,14 @@ static double pv_eq3(double x) {
     return (-5.123 * pow(x, 4) + 14.15 * pow(x, 3) - 14.21 * pow(x, 2) + 5.865 * x - 0.5696);
 }
 
-/* Simulation.cpp:875-900 (first call skips the period reduction) */
-static double pv_velocity(orc_sim* s, double x, double period, int consume_first) {
-    if (s->pv_first) {
-        if (consume_first) s->pv_first = 0;
-    } else {
-        x -= period * floor(x / period);
-    }
+/* Simulation.cpp:875-900.  The reference's static isFirstCycle flag skips the
+ * period reduction on the first call only.  A time loop that calls it once per
+ * iteration from iteration 0 (Simulation.cpp:903-932, time starting at 0.0)
+ * makes that first call the one at t = 0, so the skip is keyed to iteration 0
+ * here and on the device: the value at an iteration does not depend on which
+ * evaluation happened first. */
+static double pv_velocity(double x, double period, int64_t iter) {
+    if (iter != 0) x -= period * floor(x / period);
     double vel = 0.0;
     double nxp = fmod(x, period);
     if (nxp >= 0 && nxp < (0.1 * period)) vel = pv_eq1(nxp);
@@ -648,7 +647,7 @@ static double pv_velocity(orc_sim* s, double x, double period, int consume_first
     return vel;
 }
 
-static double wave_eval(orc_sim* s, int slot, int64_t iter, int consume) {
+static double wave_eval(const orc_sim* s, int slot, int64_t iter) {
     if (slot < 0 || slot >= MAXWAVE) return 1.0;
     const wave_t* w = &s->wave[slot];
     if (w->kind == ORC_WAVE_FOURIER) {
@@ -659,13 +658,11 @@ static double wave_eval(orc_sim* s, int slot, int64_t iter, int consume) {
         }
         return 1.0 + acc;
     }
-    if (w->kind == ORC_WAVE_PV2013) return pv_velocity(s, (double)iter * w->dt, w->period, consume);
+    if (w->kind == ORC_WAVE_PV2013) return pv_velocity((double)iter * w->dt, w->period, iter);
     return 1.0;
 }
 
-double orc_waveform_eval(const orc_sim* s, int slot, int64_t iter) {
-    return wave_eval((orc_sim*)s, slot, iter, 0);
-}
+double orc_waveform_eval(const orc_sim* s, int slot, int64_t iter) { return wave_eval(s, slot, iter); }
 
 /* One Simulation::Iterate pass (Simulation.cpp:909-918) without output. */
 static void step_once(orc_sim* s) {
@@ -687,7 +684,7 @@ static void step_once(orc_sim* s) {
     int used[MAXWAVE] = {0};
     for (int b = 0; b < s->nbc; ++b)
         if (s->bcs[b].waveform >= 0 && s->bcs[b].waveform < MAXWAVE) used[s->bcs[b].waveform] = 1;
-    for (int w = 0; w < MAXWAVE; ++w) wv[w] = used[w] ? wave_eval(s, w, s->iter, 1) : 1.0;
+    for (int w = 0; w < MAXWAVE; ++w) wv[w] = used[w] ? wave_eval(s, w, s->iter) : 1.0;
     /* Bounce-back (Simulation.cpp:1353-1370) and Zou-He (ApplyZouHeBc :941-949,
      * ApplyOutletBc :952-1348).  Each node belongs to one list and every update
      * touches only that node, so the list order of the reference is immaterial. */
@@ -748,7 +745,7 @@ static void step_once_threads(orc_sim* s, uint8_t* guard) {
     int used[MAXWAVE] = {0};
     for (int b = 0; b < s->nbc; ++b)
         if (s->bcs[b].waveform >= 0 && s->bcs[b].waveform < MAXWAVE) used[s->bcs[b].waveform] = 1;
-    for (int w = 0; w < MAXWAVE; ++w) wv[w] = used[w] ? wave_eval(s, w, s->iter, 1) : 1.0;
+    for (int w = 0; w < MAXWAVE; ++w) wv[w] = used[w] ? wave_eval(s, w, s->iter) : 1.0;
     const int rd = rest_dir(q);
     const int d0 = (s->quirks && q == 19) ? 1 : 0;
 #pragma omp parallel for schedule(static)

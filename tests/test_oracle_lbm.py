@@ -127,3 +127,16 @@ def test_threaded_step_equals_sequential(oracle, q, quirks):
     sims[0].step(40)
     sims[1].step_threads(40, 4)
     assert np.array_equal(sims[0].f(), sims[1].f())
+
+
+def test_poiseuille_config1_clean(oracle):
+    """Config 1 (32x32x128 z-pipe, R 12, tau 0.8, parabolic inlet u_max 0.05,
+    outlet rho 1.0, 2000 steps from rest) with clean bounce-back: the analytic
+    Poiseuille profile and the Hagen-Poiseuille pressure-gradient relation."""
+    from lbm_cases import assert_poiseuille, poiseuille_metrics
+    dims, R, umax, tau = (32, 32, 128), 12.0, 0.05, 0.8
+    cl, bcs = pipe_case(oracle, dims, R, umax)
+    s = oracle.Sim(19, dims, cl, tau, [oracle.make_bc(**b) for b in bcs], quirks=0)
+    s.step_threads(2000, min(8, oracle.threads_available()))
+    rho, u = s.macros()
+    assert_poiseuille(poiseuille_metrics(dims, R, tau, cl, rho, u), R, umax)

@@ -139,3 +139,34 @@ def test_c3_tube_blocks_match_oracle(ctx, oracle):
         assert np.array_equal(sub, mine), (i0, j0)
         fluid += int(sub.sum())
     assert fluid > 1000
+
+
+def test_c3_tube_512_blocks_over_the_grid_match_oracle(ctx, oracle):
+    """Config 3 at its stated size: the seeded tortuous tube (~544k facets) in
+    the 512^3 grid.  Eight 16 x 16 column blocks spread over the whole 512^2
+    grid equal the oracle's binned-mode flags bit for bit: the two corner
+    blocks hold the columns on the ray box's faces (where the ray is shifted
+    by 0.01, ComplexGeometry.cpp:334-357), three blocks sit on regular grid
+    positions and three on columns that cross the tube."""
+    from pv2013_amd import cases
+    n = 512
+    tri, g = cases.c3_mesh(seed=3, n=n)
+    print("c3 tube mesh", len(tri), "facets, grid", list(g.dims), flush=True)
+    flags, ms = ctx.voxelize(tri, g)
+    print("voxelized in %.3f ms (kernel)" % ms, flush=True)
+    vol = flags.reshape(n, n, n)
+    cols = np.argwhere(vol.any(axis=0))  # (j, i) of columns holding fluid
+    assert len(cols) > 1000
+    order = np.argsort(cols[:, 0] * n + cols[:, 1])
+    tube = [cols[order[int(f * (len(cols) - 1))]][::-1] - 8 for f in (0.2, 0.5, 0.8)]
+    picks = [np.array([0, 0]), np.array([n - 16, n - 16]), np.array([128, 384]), np.array([256, 256]),
+             np.array([384, 96])] + tube
+    fluid = 0
+    for i0, j0 in (np.clip(p, 0, n - 16).astype(int) for p in picks):
+        print("block at column (%d, %d)" % (i0, j0), flush=True)
+        og = oracle.make_grid((float(i0), float(j0), 0.0), 1.0, (16, 16, n), tuple(g.raybox))
+        sub = oracle.voxelize(tri, og, 2).reshape(n, 16, 16)
+        mine = vol[:, j0:j0 + 16, i0:i0 + 16]
+        assert np.array_equal(sub, mine), (i0, j0)
+        fluid += int(sub.sum())
+    assert fluid > 10000

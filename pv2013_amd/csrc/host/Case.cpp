@@ -35,6 +35,10 @@ bool load_case(const std::string& path, CaseConfig& c, std::string& err) {
     auto get = [&](const char* k, double def) { auto it = num.find(k); return it == num.end() ? def : it->second; };
     auto getstr = [&](const char* k) -> std::string { auto it = str.find(k); return it == str.end() ? "" : it->second; };
     c.stl_file = getstr("stl_file");
+    if (!c.stl_file.empty() && c.stl_file[0] != '/') {
+        const size_t slash = path.find_last_of('/');
+        if (slash != std::string::npos) c.stl_file = path.substr(0, slash + 1) + c.stl_file;
+    }
     c.spacing = get("spacing", c.spacing);
     c.q = (int)get("lattice", 19);
     c.quirks = (int)get("quirks", 1);
@@ -44,6 +48,10 @@ bool load_case(const std::string& path, CaseConfig& c, std::string& err) {
     c.device = (int)get("device", 0);
     if (!getstr("output_dir").empty()) c.output_dir = getstr("output_dir");
     c.autotune = get("autotune", 0) != 0;
+    c.boundary_points = get("boundary_points", 0) != 0;
+    const std::string of = getstr("output_format");
+    if (of == "raw") c.output_raw = true;
+    else if (!of.empty() && of != "ascii") { err = "output_format must be ascii or raw"; return false; }
     c.autotune_cache = getstr("autotune_cache");
     const std::string st = getstr("storage");
     if (st == "f32") c.storage = LBM_STORE_F32;

@@ -31,6 +31,12 @@ struct CaseConfig {
     long long steps = 20000;                // Simulation.cpp:908
     int output_every = 100;                 // Simulation.cpp:920
     std::string output_dir = ".";
+    // VTU files: ASCII data arrays, or one appended block of raw little-endian
+    // binary (the VTK XML "appended, raw" encoding; far smaller and faster for
+    // domains of millions of nodes)
+    bool output_raw = false;
+    // write the boundary-point files of ComplexGeometry.cpp:23-31 into output_dir
+    bool boundary_points = false;
     int device = 0;
     // lbm_autotune before building the domain: scheme, node order, kernel
     // layout and block size are timed and the fastest is used.  Every choice
@@ -39,4 +45,6 @@ struct CaseConfig {
     std::string autotune_cache;  // tuning cache file ("" = none)
 };
 
+// Relative stl_file paths are taken relative to the configuration file's
+// directory, so a case directory can be moved as a whole.
 bool load_case(const std::string& path, CaseConfig& out, std::string& err);

@@ -1,6 +1,9 @@
 #include "ComplexGeometry.h"
 
+#include <sys/stat.h>
+
 #include <cstring>
+#include <fstream>
 
 Geometry::Geometry(const CaseConfig& c, lbm_ctx* x) : cfg(c), ctx(x) {
     std::memset(&grid, 0, sizeof grid);
@@ -53,4 +56,45 @@ void Geometry::classify() {
         ok = false;
         error = lbm_last_error(ctx);
     }
+}
+
+bool Geometry::WriteBoundaryPoints(const std::string& dir, std::string& err) const {
+    mkdir(dir.c_str(), 0777);
+    const int nx = GeomSize[0], ny = GeomSize[1], nz = GeomSize[2];
+    const size_t nfiles = cfg.plane_names.size() + 1;  // planes, then the walls
+    std::vector<std::vector<int64_t>> sites(nfiles);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int64_t s = i + (int64_t)nx * (j + (int64_t)ny * k);
+                const int f = flags[s];
+                if (f == 2) sites[nfiles - 1].push_back(s);
+                else if (f >= 3 && (size_t)(f - 3) < nfiles - 1) sites[f - 3].push_back(s);
+            }
+    for (size_t p = 0; p < nfiles; ++p) {
+        const std::string name =
+            (p + 1 < nfiles ? cfg.plane_names[p] : std::string("Wall")) + "boundaryPointsFromGeometry.vtk";
+        std::ofstream out(dir + "/" + name);
+        if (!out) {
+            err = "cannot write " + dir + "/" + name;
+            return false;
+        }
+        const std::vector<int64_t>& v = sites[p];
+        out << "# vtk DataFile Version 2.0\nPoints Inside Complex Geometry\nASCII\nDATASET UNSTRUCTURED_GRID\n";
+        out << "POINTS " << v.size() << " double\n";
+        for (int64_t s : v) {
+            const int64_t idx[3] = {s % nx, (s / nx) % ny, s / ((int64_t)nx * ny)};
+            out << grid.origin[0] + idx[0] * grid.spacing << " " << grid.origin[1] + idx[1] * grid.spacing << " "
+                << grid.origin[2] + idx[2] * grid.spacing << "\n";
+        }
+        out << "CELLS " << v.size() << " " << v.size() * 2 << "\n";
+        for (size_t i = 0; i < v.size(); ++i) out << "1 " << i << "\n";
+        out << "CELL_TYPES " << v.size() << "\n";
+        for (size_t i = 0; i < v.size(); ++i) out << "1\n";
+        if (!out) {
+            err = "write failed: " + dir + "/" + name;
+            return false;
+        }
+    }
+    return true;
 }

@@ -16,6 +16,13 @@ public:
     void GetBoundingbox();
     void labelPoints();
     void classify();
+    // The reference's boundary-point files (ComplexGeometry.cpp:23-31, format of
+    // Writer::writePointsToFile, GeometryWriter.cpp:35-68): one legacy VTK file
+    // per inlet/outlet plane, <name>boundaryPointsFromGeometry.vtk, and
+    // WallboundaryPointsFromGeometry.vtk, points in i-j-k order (i outermost)
+    // at the grid coordinates origin + index * spacing.  A point lies in the file
+    // of its first matching plane (the node priority of Subdomain.cpp:413-473).
+    bool WriteBoundaryPoints(const std::string& dir, std::string& err) const;
 
     ::Reader reader;
     std::vector<Triangle> triangles;

@@ -20,6 +20,13 @@
 //   waveform <slot> pv2013 <dt>
 //   lattice 19|27   storage f64|f32|f16s   scheme two_lattice|aa   quirks 1|0
 //   tau <tau>   steps <n>   output_every <n>   output_dir <dir>   device <id>
+//   output_format ascii|raw    VTU data arrays as text or appended raw binary
+//   boundary_points 1          also write the boundary-point files of
+//                              ComplexGeometry.cpp:23-31 (<plane>boundaryPoints
+//                              FromGeometry.vtk per plane, WallboundaryPoints
+//                              FromGeometry.vtk)
+//   autotune 1   autotune_cache <file>
+// A relative stl_file is read from the configuration file's directory.
 #include <chrono>
 #include <cstdio>
 
@@ -56,6 +63,14 @@ int main(int argc, char** argv) {
         std::printf("grid %d x %d x %d, %zu facets, voxelize %.3f ms, classify %.3f ms\n", geometry.GeomSize[0],
                     geometry.GeomSize[1], geometry.GeomSize[2], geometry.triangles.size(), geometry.voxelize_ms,
                     geometry.classify_ms);
+        if (cfg.boundary_points) {
+            std::string werr;
+            if (!geometry.WriteBoundaryPoints(cfg.output_dir, werr)) {
+                std::fprintf(stderr, "geometry: %s\n", werr.c_str());
+                lbm_ctx_destroy(ctx);
+                return 1;
+            }
+        }
         SubDomain subdomain(geometry, cfg, ctx);
         if (!subdomain.ok) {
             std::fprintf(stderr, "domain: %s\n", subdomain.error.c_str());
@@ -69,7 +84,7 @@ int main(int argc, char** argv) {
                         subdomain.tuned.scheme, subdomain.tuned.fast_axis, subdomain.tuned.options,
                         subdomain.tuned.block, subdomain.tuned.ntried, subdomain.tuned.mlups);
         Simulation sim(subdomain, cfg.tau, 1.0, 0.5e-5);
-        if (!sim.Iterate(cfg.steps, cfg.output_every, cfg.output_dir)) {
+        if (!sim.Iterate(cfg.steps, cfg.output_every, cfg.output_dir, cfg.output_raw)) {
             std::fprintf(stderr, "simulation: %s\n", sim.error.c_str());
             rc = 1;
         } else {

@@ -21,6 +21,11 @@ public:
     // i-j-k ordered wall sites (i outermost), the WallBoundaryNode_ order of
     // Subdomain.cpp:464-469
     std::vector<int64_t> wall_sites;
+    // The node fields the reference's Collision sets (Simulation.cpp:21-179:
+    // Node::density, Node::velocity, the moments of the state the collision
+    // starts from) and ComputeWallShearStress (Simulation.cpp:1406-1421) sets
+    // from them; dense box arrays, filled by Simulation::CollectNodeFields.
+    std::vector<double> density, velocity, wall_shear_stress;
     bool ok = true;
     std::string error;
     // the autotuner's choice when the configuration asked for one (ntried = -1: not tuned)

@@ -328,13 +328,29 @@ def fourier_waveform(seed, nk=4, amp=0.5):
     return a, phi
 
 
-def write_stl_ascii(path, tri, name="vessel"):
+def write_stl_ascii(path, tri, name="vessel", exact=False):
+    """ASCII STL in the layout Reader::readSTLASCII parses (GeometryReader.cpp:
+    33-117).  exact: 17 significant digits, so every double reads back
+    unchanged; otherwise 10."""
+    f = "%.17g" if exact else "%.9e"
+    row = ("facet normal {0} {0} {0}\n outer loop\n  vertex {0} {0} {0}\n  vertex {0} {0} {0}\n"
+           "  vertex {0} {0} {0}\n endloop\nendfacet\n").format(f)
     with open(path, "w") as fh:
         fh.write("solid %s\n" % name)
-        lines = []
-        for t in tri:
-            lines.append("facet normal %.9e %.9e %.9e\n outer loop\n"
-                         "  vertex %.9e %.9e %.9e\n  vertex %.9e %.9e %.9e\n  vertex %.9e %.9e %.9e\n"
-                         " endloop\nendfacet\n" % (t[9], t[10], t[11], *t[:9]))
-        fh.write("".join(lines))
+        fh.write("".join(row % (t[9], t[10], t[11], *t[:9]) for t in tri))
         fh.write("endsolid %s\n" % name)
+
+
+def write_stl_binary(path, tri, name="vessel"):
+    """Binary STL (the layout Reader::readSTLBinary reads, GeometryReader.cpp:
+    119-176): an 80-byte header, a uint32 facet count, then per facet the normal
+    and three vertices as little-endian float32 and a zero uint16.  Values are
+    rounded to float32."""
+    tri = np.asarray(tri, np.float64)
+    rec = np.zeros(len(tri), dtype=[("n", "<f4", 3), ("v", "<f4", 9), ("attr", "<u2")])
+    rec["n"] = tri[:, 9:12]
+    rec["v"] = tri[:, :9]
+    with open(path, "wb") as fh:
+        fh.write(("binary STL " + name).encode()[:80].ljust(80, b" "))
+        fh.write(np.uint32(len(tri)).tobytes())
+        fh.write(rec.tobytes())

@@ -19,8 +19,14 @@ Output arrays:
              inlet, out1..out6, wall
   dims       grid dims implied by the bbox (ComplexGeometry.cpp:68-72)
 
+and tests/golden/pv2013_points_sha256.json: the SHA-256 digest of each of the
+eight boundary-point files, so that files written by the engine or the oracle
+can be checked for byte identity with the reference's.
+
 Run:  python tests/golden/make_pv2013_fixtures.py [/root/reference]
 """
+import hashlib
+import json
 import math
 import os
 import sys
@@ -92,6 +98,11 @@ def main():
         out[key] = np.sort(site)
         print(key, len(site))
     print("max fractional index error", worst)
+    digests = {fname: hashlib.sha256(open(os.path.join(ref, fname), "rb").read()).hexdigest()
+               for fname in names.values()}
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "pv2013_points_sha256.json"), "w") as fh:
+        json.dump(digests, fh, indent=1, sort_keys=True)
+        fh.write("\n")
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pv2013.npz")
     np.savez_compressed(dst, **out)
     print("wrote", dst, os.path.getsize(dst), "bytes")

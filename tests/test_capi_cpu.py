@@ -74,6 +74,25 @@ def test_stl_binary_renormalises_in_float(P, tmp_path):
     assert np.array_equal(got[:, 9:], (n / norm[:, None]).astype(np.float32).astype(np.float64))
 
 
+def test_generated_stl_writers_round_trip(P, tmp_path):
+    """The generators' writers: an exact ASCII file reads back to the same
+    doubles; a binary file (GeometryReader.cpp:119-176 layout) reads back to the
+    float32 vertices, with the float32 normals renormalised in float as the
+    reference's reader does."""
+    from pv2013_amd.geometry import bent_pipe, write_stl_ascii, write_stl_binary
+    tri, _ = bent_pipe(30, 5, 200.0)
+    a, b = str(tmp_path / "a.stl"), str(tmp_path / "b.stl")
+    write_stl_ascii(a, tri, exact=True)
+    write_stl_binary(b, tri)
+    assert np.array_equal(P.read_stl(a), tri)
+    got = P.read_stl(b)
+    assert got.shape == tri.shape and os.path.getsize(b) == 84 + 50 * len(tri)
+    assert np.array_equal(got[:, :9], tri[:, :9].astype(np.float32).astype(np.float64))
+    n = tri[:, 9:].astype(np.float32)
+    norm = np.sqrt(n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1] + n[:, 2] * n[:, 2]).astype(np.float32)
+    assert np.array_equal(got[:, 9:], (n / norm[:, None]).astype(np.float32).astype(np.float64))
+
+
 def test_header_compiles_as_c99(tmp_path):
     """include/lbm_capi.h is plain C: a C99 translation unit that takes the
     address of every declared function compiles with -pedantic -Werror (what a

@@ -1,5 +1,6 @@
 """lbm_run end to end: STL + configuration -> GPU voxelizer -> steps -> VTU
-files, identical to the same case run through the C ABI from Python."""
+files, against orc_run (the CPU oracle driven by the same configuration file)
+file for file."""
 import os
 import re
 import subprocess
@@ -8,52 +9,52 @@ import numpy as np
 import pytest
 
 from host_case import write_case
+from test_oracle_driver import case_with
+from vtu import read_vtu
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RUN = os.path.join(ROOT, "pv2013_amd", "csrc", "lbm_run")
+ORC_RUN = os.path.join(ROOT, "oracle", "orc_run")
 
 
-def vtu_array(path, name):
-    text = open(path).read()
-    m = re.search(r'Name="%s"[^>]*>(.*?)</DataArray>' % name, text, re.S)
-    return np.array([float(v) for v in m.group(1).split()])
-
-
-def test_lbm_run_matches_python_path(tmp_path):
-    import pv2013_amd as P
-    cfg, stl, info = write_case(str(tmp_path))
-    print("lbm_run", cfg, flush=True)
-    r = subprocess.run([RUN, cfg], capture_output=True, text=True, timeout=120)
+def run(exe, cfg, timeout=300):
+    r = subprocess.run([exe, cfg], capture_output=True, text=True, timeout=timeout)
     assert r.returncode == 0, r.stdout + r.stderr
-    out = os.path.join(str(tmp_path), "out")
-    for f in ("Fluid_Variables/0_rank_0.vtu", "Fluid_Variables/100_rank_0.vtu",
-              "Wall_Shear_Stress_VTK/100_rank_0_wall_shear_stress.vtu"):
-        assert os.path.exists(os.path.join(out, f)), f
-    ctx = P.Context(0)
-    tri = P.read_stl(stl)
-    dims = info["dims"]
-    g = P.make_grid((0, 0, 0), 1.0, dims, (0, dims[0] - 1, 0, dims[1] - 1, 0, dims[2] - 1))
-    flags, _ = ctx.voxelize(tri, g)
-    flags, _ = ctx.classify(g, flags, [P.make_plane(2, "1"), P.make_plane(2, str(info["outlet_z"]))])
-    cx, cy = info["inlet_center"]
-    bcs = [P.make_bc(P.BC_VELOCITY, 2, 1, u=(0, 0, 0.03), profile=P.PROFILE_PARABOLIC, center=(cx, cy),
-                     radius=float(info["radius"])),
-           P.make_bc(P.BC_PRESSURE, 2, -1, rho=1.0)]
-    dom = P.Domain(ctx, dims, flags, 0.7, bcs)
-    dom.step(101)
-    rho, u = dom.macros()
-    fl = flags > 0
-    vtu = os.path.join(out, "Fluid_Variables/100_rank_0.vtu")
-    assert np.array_equal(vtu_array(vtu, "Density"), rho[fl])
-    assert np.array_equal(vtu_array(vtu, "Velocity").reshape(-1, 3), u[:, fl].T)
-    wss = dom.wss()
-    w_vtu = vtu_array(os.path.join(out, "Wall_Shear_Stress_VTK/100_rank_0_wall_shear_stress.vtu"), "WallShearStress")
-    walls = np.nonzero(flags == 2)[0]
-    nx, ny, nz = dims
-    k, rem = walls // (nx * ny), walls % (nx * ny)
-    order = np.lexsort((k, rem // nx, rem % nx))  # i outermost, then j, then k
-    assert np.array_equal(w_vtu, wss[walls[order]])
+    return r.stdout
+
+
+def same_outputs(a, b, iters, points_files=()):
+    """Every VTU array of iteration n bit for bit, and the boundary-point files
+    byte for byte, between the output folders a and b."""
+    for n in iters:
+        for f in ("Fluid_Variables/%d_rank_0.vtu" % n, "Wall_Shear_Stress_VTK/%d_rank_0_wall_shear_stress.vtu" % n):
+            x, y = read_vtu(os.path.join(a, f)), read_vtu(os.path.join(b, f))
+            assert sorted(x) == sorted(y), f
+            for k in x:
+                assert x[k].dtype == y[k].dtype and np.array_equal(x[k].view(np.uint8), y[k].view(np.uint8)), (f, k)
+    for f in points_files:
+        assert open(os.path.join(a, f), "rb").read() == open(os.path.join(b, f), "rb").read(), f
+
+
+@pytest.mark.parametrize("fmt", ["ascii", "raw"])
+def test_lbm_run_matches_orc_run(tmp_path, fmt):
+    """One case file drives the GPU engine (lbm_run) and the CPU oracle
+    (orc_run): FP64 with the reference's quirks, a steady parabolic inlet, a
+    pressure outlet; the files of iterations 0, 50 and 100 (Velocity, Density,
+    WSS of the state each iteration collides, Pressure after it) and the
+    boundary-point files agree bit for bit."""
+    d = str(tmp_path)
+    cfg, _, _ = write_case(d, steps=101, every=50)
+    gpu = case_with(os.path.join(d, "gpu.cfg"), cfg, output_dir=os.path.join(d, "gpu"), output_format=fmt,
+                    boundary_points=1)
+    cpu = case_with(os.path.join(d, "cpu.cfg"), cfg, output_dir=os.path.join(d, "cpu"), output_format=fmt,
+                    boundary_points=1)
+    print("lbm_run / orc_run", fmt, flush=True)
+    run(RUN, gpu)
+    run(ORC_RUN, cpu)
+    same_outputs(os.path.join(d, "gpu"), os.path.join(d, "cpu"), (0, 50, 100),
+                 ["%sboundaryPointsFromGeometry.vtk" % p for p in ("inlet", "outlet", "Wall")])
 
 
 def test_lbm_run_autotune_gives_identical_files(tmp_path):

@@ -89,7 +89,8 @@ int lbm_grid_from_bbox(const double* tri, int64_t ntri, double spacing, lbm_grid
 
 /* Geometry::labelPoints (ComplexGeometry.cpp:431-506): +z ray parity per grid
  * point, bit-exact.  flags_out gets 0 (outside) or 1 (inside).  kernel_ms
- * (may be NULL) receives the device time of the voxelizer kernels. */
+ * (may be NULL) receives the device time of the facet-binning and voxelizer
+ * kernels (host copies excluded). */
 int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const lbm_grid_desc* grid,
                  uint8_t* flags_out, double* kernel_ms);
 

@@ -8,13 +8,13 @@
 // the count is odd.
 //
 // The XY test does not depend on z, so the work is organised per column:
-//   host   facets with nz == 0 are dropped (their t is +-inf or NaN, never in
+//   bins   facets with nz == 0 are dropped (their t is +-inf or NaN, never in
 //          the window); facets whose XY projection is thinner than ~1e-6 rad
 //          go to a list every column tests; the rest are binned into 8 x 8
 //          column tiles by their XY bbox widened by a margin that bounds the
 //          rounding error of the sign test (so binning never loses a facet the
-//          exact test would accept);
-//   device one workgroup of four wavefronts per tile (tiles with the longest
+//          exact test would accept) -- three device kernels, see k_bin_count;
+//   voxels one workgroup of four wavefronts per tile (tiles with the longest
 //          lists launch first), one lane per column in each wavefront; the
 //          wavefronts take the tile's facets in turns of 64, staged through
 //          LDS; for each facet that passes the exact XY test the lane toggles
@@ -30,7 +30,6 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <thread>
 #include <vector>
 
 #include "lbm_internal.h"
@@ -39,6 +38,13 @@
 int lbm_ctx_fail(lbm_ctx* ctx, int code, const std::string& msg);
 hipStream_t lbm_ctx_stream(lbm_ctx* ctx);
 int lbm_ctx_device(lbm_ctx* ctx);
+
+#define VCHECK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return lbm_ctx_fail(ctx, LBM_EIO, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
 
 namespace {
 
@@ -224,165 +230,300 @@ struct DevMem {
 
 // Sliver test: a facet whose XY projection has its smallest angle below about
 // 1e-6 rad is tested at every column instead of being binned.
-bool thin_xy(const double* T) {
+__device__ bool thin_xy(const double* T) {
     const double ax = T[3] - T[0], ay = T[4] - T[1], bx = T[6] - T[0], by = T[7] - T[1];
     const double cx = T[6] - T[3], cy = T[7] - T[4];
-    const double area2 = std::fabs(ax * by - ay * bx);
-    const double l = std::max(ax * ax + ay * ay, std::max(bx * bx + by * by, cx * cx + cy * cy));
+    const double area2 = fabs(ax * by - ay * bx);
+    const double l = fmax(ax * ax + ay * ay, fmax(bx * bx + by * by, cx * cx + cy * cy));
     return !(area2 >= 1e-6 * l) || l == 0.0;
 }
 
-}  // namespace
+// ---------------------------------------------------------- facet binning
+//
+// On the device, in three launches:
+//   k_bin_count  one thread per facet: facets with nz == 0 are dropped (their
+//                t is +-inf or NaN, never in the window); slivers (thin_xy) are
+//                appended to the list every column tests; the rest get the
+//                range of 8 x 8 column tiles their XY bbox touches, widened by
+//                a margin that bounds the rounding error of the sign test plus
+//                the ray's 0.01 shift at the ray box faces (binning never
+//                loses a facet the exact test would accept), and count into
+//                each of those tiles;
+//   k_bin_scan   one workgroup: tile list offsets (exclusive prefix sum), the
+//                fill cursors, and the tile launch order (tiles bucketed by
+//                floor(log2(list length)), longest buckets first, so the
+//                slowest workgroups start first);
+//   k_bin_fill   one thread per facet: appends the facet to each tile's list.
+// List order within a tile varies from run to run; the flags do not, since a
+// column's parity is the xor of independent toggles.
 
-#define VCHECK(call)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return lbm_ctx_fail(ctx, LBM_EIO, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+struct BinBufs {
+    unsigned long long* hdr;  // [0] list entries, [1] facets every column tests, [2] entries above 2^31 - 1
+    int32_t* cnt;             // per tile: count, then the fill cursor
+    int32_t* off;             // ntiles + 1 list offsets
+    int32_t* order;           // tiles in launch order
+    int4* rng;                // per facet: tile range x0, x1, y0, y1 (x0 > x1: none)
+    int32_t* glob;            // facets every column tests
+    int32_t* list;            // the tile lists
+    int64_t list_cap;
+};
+
+struct BinArgs {
+    const double* tri;
+    int64_t ntri, ntiles;
+    int32_t nx, ny, ntx;
+    double ox, oy, sp, slack;
+    BinBufs b;
+};
+
+constexpr int HDR_WORDS = 4;
+constexpr int SCAN_THREADS = 1024;
+constexpr int NBUCKETS = 33;  // list length 0, then [2^(b-1), 2^b)
+
+__global__ __launch_bounds__(256) void k_bin_zero(uint32_t* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntri) return;
+    const double* T = a.tri + 12 * t;
+    int4 r = make_int4(1, 0, 1, 0);
+    if (T[11] != 0.0) {
+        if (thin_xy(T)) {
+            const unsigned long long pos = atomicAdd(&a.b.hdr[1], 1ull);
+            a.b.glob[pos] = (int32_t)t;
+        } else {
+            const double xlo = fmin(T[0], fmin(T[3], T[6])) - a.slack, xhi = fmax(T[0], fmax(T[3], T[6])) + a.slack;
+            const double ylo = fmin(T[1], fmin(T[4], T[7])) - a.slack, yhi = fmax(T[1], fmax(T[4], T[7])) + a.slack;
+            const double i0 = fmax(floor((xlo - a.ox) / a.sp) - 1.0, 0.0);
+            const double i1 = fmin(ceil((xhi - a.ox) / a.sp) + 1.0, (double)(a.nx - 1));
+            const double j0 = fmax(floor((ylo - a.oy) / a.sp) - 1.0, 0.0);
+            const double j1 = fmin(ceil((yhi - a.oy) / a.sp) + 1.0, (double)(a.ny - 1));
+            if (i0 <= i1 && j0 <= j1) {
+                r = make_int4((int)i0 / TX, (int)i1 / TX, (int)j0 / TY, (int)j1 / TY);
+                for (int ty = r.z; ty <= r.w; ++ty)
+                    for (int tx = r.x; tx <= r.y; ++tx) atomicAdd(&a.b.cnt[(int64_t)ty * a.ntx + tx], 1);
+                atomicAdd(&a.b.hdr[0], (unsigned long long)(r.y - r.x + 1) * (unsigned long long)(r.w - r.z + 1));
+            }
+        }
+    }
+    a.b.rng[t] = r;
+}
+
+__device__ __forceinline__ int length_bucket(int len) { return len <= 0 ? 0 : 32 - __clz(len); }
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(BinArgs a) {
+    __shared__ long long part[SCAN_THREADS];
+    __shared__ int bcnt[NBUCKETS], bcur[NBUCKETS];
+    const int tid = threadIdx.x;
+    const int64_t n = a.ntiles;
+    const int64_t c0 = n * tid / SCAN_THREADS, c1 = n * (tid + 1) / SCAN_THREADS;
+    if (tid < NBUCKETS) bcnt[tid] = 0;
+    long long sum = 0;
+    for (int64_t i = c0; i < c1; ++i) sum += a.b.cnt[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int h = 1; h < SCAN_THREADS; h <<= 1) {
+        const long long v = tid >= h ? part[tid - h] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    long long run = part[tid] - sum;
+    for (int64_t i = c0; i < c1; ++i) {
+        const int len = a.b.cnt[i];
+        const int32_t o = (int32_t)(run < INT32_MAX ? run : INT32_MAX);
+        a.b.off[i] = o;
+        a.b.cnt[i] = o;  // the fill cursor
+        atomicAdd(&bcnt[length_bucket(len)], 1);
+        run += len;
+    }
+    if (tid == SCAN_THREADS - 1) {
+        a.b.off[n] = (int32_t)(run < INT32_MAX ? run : INT32_MAX);
+        a.b.hdr[2] = run > INT32_MAX ? 1ull : 0ull;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int b = NBUCKETS - 1; b >= 0; --b) {
+            bcur[b] = acc;
+            acc += bcnt[b];
+        }
+    }
+    __syncthreads();
+    for (int64_t i = c0; i < c1; ++i) {
+        // offsets saturated at 2^31 - 1 (then the call is rejected) could change
+        // a length's bucket: never write past the order array
+        const int pos = atomicAdd(&bcur[length_bucket(a.b.off[i + 1] - a.b.off[i])], 1);
+        if (pos < n) a.b.order[pos] = (int32_t)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntri) return;
+    const int4 r = a.b.rng[t];
+    for (int ty = r.z; ty <= r.w; ++ty)
+        for (int tx = r.x; tx <= r.y; ++tx) {
+            const int32_t pos = atomicAdd(&a.b.cnt[(int64_t)ty * a.ntx + tx], 1);
+            if (pos < a.b.list_cap) a.b.list[pos] = (int32_t)t;
+        }
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Bytes of the binning buffers other than the tile lists, laid out from base
+// (every array 256-byte aligned); fills b when base is not null.
+size_t bin_layout(int64_t ntri, int64_t ntiles, char* base, BinBufs* b) {
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return base ? base + at : nullptr;
+    };
+    char* hdr = take(sizeof(unsigned long long) * HDR_WORDS);
+    char* cnt = take(sizeof(int32_t) * (size_t)ntiles);
+    char* off = take(sizeof(int32_t) * (size_t)(ntiles + 1));
+    char* order = take(sizeof(int32_t) * (size_t)ntiles);
+    char* rng = take(sizeof(int4) * (size_t)std::max<int64_t>(ntri, 1));
+    char* glob = take(sizeof(int32_t) * (size_t)std::max<int64_t>(ntri, 1));
+    if (b) {
+        b->hdr = (unsigned long long*)hdr;
+        b->cnt = (int32_t*)cnt;
+        b->off = (int32_t*)off;
+        b->order = (int32_t*)order;
+        b->rng = (int4*)rng;
+        b->glob = (int32_t*)glob;
+        b->list = nullptr;
+        b->list_cap = 0;
+    }
+    return o;
+}
+
+// Grid checks shared by the voxelizer entry points.
+int check_grid(lbm_ctx* ctx, const char* fn, int64_t ntri, const lbm_grid_desc* g) {
+    const int nx = g->dims[0], ny = g->dims[1], nz = g->dims[2];
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(g->spacing > 0))
+        return lbm_ctx_fail(ctx, LBM_EINVAL, std::string(fn) + ": bad grid");
+    if (nz > MAXNZ) return lbm_ctx_fail(ctx, LBM_ERANGE, std::string(fn) + ": nz above 2048");
+    if (ntri > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, std::string(fn) + ": too many facets");
+    const int64_t ntiles = (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY);
+    if (ntiles > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, std::string(fn) + ": more than 2^31 column tiles");
+    return LBM_OK;
+}
+
+// Bins the device-resident facets d_tri and voxelizes into the device flags
+// d_flags on the context stream.  The binning buffers are `b` (bin_layout);
+// the tile lists come from list_buf(entries), which returns a device buffer of
+// at least that many int32 or null (an error already recorded).  *ms gets the
+// device time from the first binning kernel to the end of the voxelizer.
+template <class ListBuf>
+int bin_and_voxelize(lbm_ctx* ctx, const char* fn, const double* d_tri, int64_t ntri, const lbm_grid_desc* g,
+                     uint8_t* d_flags, BinBufs b, ListBuf&& list_buf, double* ms) {
+    hipStream_t st = lbm_ctx_stream(ctx);
+    const int nx = g->dims[0], ny = g->dims[1], nz = g->dims[2];
+    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
+    const int64_t ntiles = (int64_t)ntx * nty;
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a)
+        ext = std::max(ext, std::fabs(g->raybox[2 * a + 1] - g->raybox[2 * a]) + std::fabs(g->origin[a]) +
+                                g->dims[a] * g->spacing);
+    BinArgs ba;
+    ba.tri = d_tri;
+    ba.ntri = ntri;
+    ba.ntiles = ntiles;
+    ba.nx = nx;
+    ba.ny = ny;
+    ba.ntx = ntx;
+    ba.ox = g->origin[0];
+    ba.oy = g->origin[1];
+    ba.sp = g->spacing;
+    ba.slack = 1e-7 * std::max(ext, 1.0) + g->ray_offset;
+    ba.b = b;
+    EventPair ev;
+    VCHECK(hipEventCreate(&ev.e0));
+    VCHECK(hipEventCreate(&ev.e1));
+    VCHECK(hipEventRecord(ev.e0, st));
+    // counters left by the previous call start at zero (a fill kernel, no memset node)
+    const int64_t zw = (int64_t)((char*)b.off - (char*)b.hdr) / 4;  // hdr and cnt
+    hipLaunchKernelGGL(k_bin_zero, dim3((unsigned)((zw + 255) / 256)), dim3(256), 0, st, (uint32_t*)b.hdr, zw);
+    VCHECK(hipGetLastError());
+    const unsigned fgrid = (unsigned)std::max<int64_t>(1, (ntri + 255) / 256);
+    if (ntri > 0) {
+        hipLaunchKernelGGL(k_bin_count, dim3(fgrid), dim3(256), 0, st, ba);
+        VCHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(SCAN_THREADS), 0, st, ba);
+    VCHECK(hipGetLastError());
+    unsigned long long hdr[HDR_WORDS];
+    VCHECK(hipMemcpyAsync(hdr, b.hdr, sizeof hdr, hipMemcpyDeviceToHost, st));
+    VCHECK(hipStreamSynchronize(st));
+    if (hdr[2] || hdr[0] > (unsigned long long)INT32_MAX)
+        return lbm_ctx_fail(ctx, LBM_ERANGE, std::string(fn) + ": bin lists too long");
+    ba.b.list = list_buf((int64_t)hdr[0]);
+    if (!ba.b.list) return LBM_ERANGE;
+    ba.b.list_cap = (int64_t)hdr[0];
+    if (ntri > 0) {
+        hipLaunchKernelGGL(k_bin_fill, dim3(fgrid), dim3(256), 0, st, ba);
+        VCHECK(hipGetLastError());
+    }
+    VoxArgs a;
+    a.tri = d_tri; a.tile_off = b.off; a.order = b.order; a.tile_tri = ba.b.list; a.global_tri = b.glob;
+    a.nglobal = (int32_t)hdr[1];
+    a.nx = nx; a.ny = ny; a.nz = nz; a.ntx = ntx;
+    a.ox = g->origin[0]; a.oy = g->origin[1]; a.oz = g->origin[2]; a.sp = g->spacing;
+    a.bminx = g->raybox[0]; a.bmaxx = g->raybox[1]; a.bminy = g->raybox[2]; a.bmaxy = g->raybox[3];
+    a.L = g->raybox[5] - g->raybox[4];
+    a.pad = g->ray_pad; a.off = g->ray_offset; a.teps = g->t_eps;
+    a.flags = d_flags;
+    const size_t lds_bits = sizeof(uint32_t) * 64 * (size_t)((nz + 31) / 32);
+    hipLaunchKernelGGL(k_voxelize, dim3((unsigned)ntiles), dim3(64 * VWAVES), lds_bits, st, a);
+    VCHECK(hipGetLastError());
+    VCHECK(hipEventRecord(ev.e1, st));
+    VCHECK(hipEventSynchronize(ev.e1));
+    float t = 0.f;
+    VCHECK(hipEventElapsedTime(&t, ev.e0, ev.e1));
+    if (ms) *ms = t;
+    return LBM_OK;
+}
+
+}  // namespace
 
 extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const lbm_grid_desc* g,
                             uint8_t* flags_out, double* kernel_ms) {
     if (!ctx) return LBM_EINVAL;
     if (!g || !flags_out || (ntri > 0 && !tri) || ntri < 0)
         return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_voxelize: null argument");
-    const int nx = g->dims[0], ny = g->dims[1], nz = g->dims[2];
-    if (nx <= 0 || ny <= 0 || nz <= 0 || !(g->spacing > 0))
-        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_voxelize: bad grid");
-    if (nz > MAXNZ) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: nz above 2048");
-    if (ntri > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: too many facets");
+    if (int rc = check_grid(ctx, "lbm_voxelize", ntri, g)) return rc;
     VCHECK(hipSetDevice(lbm_ctx_device(ctx)));
     hipStream_t st = lbm_ctx_stream(ctx);
-    const double ox = g->origin[0], oy = g->origin[1], sp = g->spacing;
-    double ext = 0.0;
-    for (int a = 0; a < 3; ++a)
-        ext = std::max(ext, std::fabs(g->raybox[2 * a + 1] - g->raybox[2 * a]) + std::fabs(g->origin[a]) +
-                                g->dims[a] * g->spacing);
-    const double margin = 1e-7 * std::max(ext, 1.0);
-    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
-    const int64_t ntiles = (int64_t)ntx * nty;
-    if (ntiles > INT32_MAX) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: more than 2^31 column tiles");
-    // Bin facets into column tiles.  Up to 16 host threads take contiguous facet
-    // ranges with per-thread tile counts; the counts then become per-thread
-    // write offsets, so every tile's list holds its facets in facet order, as a
-    // single pass would (the flags do not depend on the order either way).
-    struct Part {
-        std::vector<int32_t> keep, global, cnt;
-        std::vector<std::array<int32_t, 4>> rng;
-    };
-    const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
-    int nthr = (int)std::min<int64_t>(std::min<int64_t>(16, hw), ntri / 16384 + 1);
-    nthr = (int)std::max<int64_t>(1, std::min<int64_t>(nthr, ((int64_t)1 << 26) / ntiles));
-    std::vector<Part> parts(nthr);
-    bool oom = false;
-    auto run = [&](auto&& work) {
-        std::vector<std::thread> th;
-        std::vector<char> failed(nthr, 0);
-        auto guarded = [&](int p) {
-            try {
-                work(p);
-            } catch (...) {
-                failed[p] = 1;
-            }
-        };
-        for (int p = 1; p < nthr; ++p) th.emplace_back(guarded, p);
-        guarded(0);
-        for (auto& t : th) t.join();
-        for (char f : failed) oom = oom || f;
-    };
-    const double slack = margin + g->ray_offset;
-    run([&](int p) {
-        Part& P = parts[p];
-        P.cnt.assign((size_t)ntiles, 0);
-        for (int64_t t = ntri * p / nthr; t < ntri * (p + 1) / nthr; ++t) {
-            const double* T = tri + 12 * t;
-            if (T[11] == 0.0) continue;  // t = +-inf or NaN: never counted
-            if (thin_xy(T)) { P.global.push_back((int32_t)t); continue; }
-            const double xlo = std::min({T[0], T[3], T[6]}) - slack, xhi = std::max({T[0], T[3], T[6]}) + slack;
-            const double ylo = std::min({T[1], T[4], T[7]}) - slack, yhi = std::max({T[1], T[4], T[7]}) + slack;
-            long i0 = (long)std::floor((xlo - ox) / sp) - 1, i1 = (long)std::ceil((xhi - ox) / sp) + 1;
-            long j0 = (long)std::floor((ylo - oy) / sp) - 1, j1 = (long)std::ceil((yhi - oy) / sp) + 1;
-            i0 = std::max(i0, 0L); j0 = std::max(j0, 0L);
-            i1 = std::min(i1, (long)nx - 1); j1 = std::min(j1, (long)ny - 1);
-            if (i0 > i1 || j0 > j1) continue;
-            std::array<int32_t, 4> r = {(int32_t)(i0 / TX), (int32_t)(i1 / TX), (int32_t)(j0 / TY),
-                                        (int32_t)(j1 / TY)};
-            P.keep.push_back((int32_t)t);
-            P.rng.push_back(r);
-            for (int ty = r[2]; ty <= r[3]; ++ty)
-                for (int tx = r[0]; tx <= r[1]; ++tx) P.cnt[(int64_t)ty * ntx + tx]++;
-        }
-    });
-    if (oom) return lbm_ctx_fail(ctx, LBM_ENOMEM, "lbm_voxelize: out of host memory while binning");
-    // tile offsets, then each thread's write offset inside every tile
-    std::vector<int32_t> cnt(ntiles + 1, 0);
-    int64_t total = 0;
-    for (int64_t i = 0; i < ntiles; ++i) {
-        cnt[i] = (int32_t)std::min<int64_t>(total, INT32_MAX);
-        for (int p = 0; p < nthr; ++p) {
-            const int32_t c = parts[p].cnt[i];
-            parts[p].cnt[i] = (int32_t)std::min<int64_t>(total, INT32_MAX);
-            total += c;
-        }
-    }
-    if (total > INT32_MAX / 2) return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize: bin lists too long");
-    cnt[ntiles] = (int32_t)total;
-    std::vector<int32_t> lists(std::max<int64_t>(total, 1)), global;
-    for (const Part& P : parts) global.insert(global.end(), P.global.begin(), P.global.end());
-    // longest lists first, so the slowest workgroups start first
-    std::vector<int32_t> order(ntiles);
-    for (int64_t i = 0; i < ntiles; ++i) order[i] = (int32_t)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t x, int32_t y) { return cnt[x + 1] - cnt[x] > cnt[y + 1] - cnt[y]; });
-    run([&](int p) {
-        Part& P = parts[p];
-        for (size_t u = 0; u < P.keep.size(); ++u) {
-            const auto& r = P.rng[u];
-            for (int ty = r[2]; ty <= r[3]; ++ty)
-                for (int tx = r[0]; tx <= r[1]; ++tx) lists[P.cnt[(int64_t)ty * ntx + tx]++] = P.keep[u];
-        }
-    });
+    const int nx = g->dims[0], ny = g->dims[1], nz = g->dims[2];
+    const int64_t ntiles = (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY);
+    const int64_t nsites = (int64_t)nx * ny * nz;
     DevMem m;
     double* d_tri;
-    int32_t *d_off, *d_list, *d_glob, *d_order;
     uint8_t* d_flags;
-    const int64_t nsites = (int64_t)nx * ny * nz;
+    char* d_bins;
     VCHECK(m.alloc(&d_tri, (size_t)std::max<int64_t>(ntri, 1) * 12));
-    VCHECK(m.alloc(&d_off, (size_t)ntiles + 1));
-    VCHECK(m.alloc(&d_order, (size_t)ntiles));
-    VCHECK(m.alloc(&d_list, lists.size()));
-    VCHECK(m.alloc(&d_glob, std::max<size_t>(global.size(), 1)));
     VCHECK(m.alloc(&d_flags, (size_t)nsites));
+    VCHECK(m.alloc(&d_bins, bin_layout(ntri, ntiles, nullptr, nullptr)));
+    BinBufs b;
+    bin_layout(ntri, ntiles, d_bins, &b);
     if (ntri > 0) VCHECK(hipMemcpyAsync(d_tri, tri, sizeof(double) * 12 * ntri, hipMemcpyHostToDevice, st));
-    VCHECK(hipMemcpyAsync(d_off, cnt.data(), sizeof(int32_t) * (ntiles + 1), hipMemcpyHostToDevice, st));
-    VCHECK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * ntiles, hipMemcpyHostToDevice, st));
-    VCHECK(hipMemcpyAsync(d_list, lists.data(), sizeof(int32_t) * lists.size(), hipMemcpyHostToDevice, st));
-    if (!global.empty())
-        VCHECK(hipMemcpyAsync(d_glob, global.data(), sizeof(int32_t) * global.size(), hipMemcpyHostToDevice, st));
-    VoxArgs a;
-    a.tri = d_tri; a.tile_off = d_off; a.order = d_order; a.tile_tri = d_list; a.global_tri = d_glob;
-    a.nglobal = (int32_t)global.size();
-    a.nx = nx; a.ny = ny; a.nz = nz; a.ntx = ntx;
-    a.ox = ox; a.oy = oy; a.oz = g->origin[2]; a.sp = sp;
-    a.bminx = g->raybox[0]; a.bmaxx = g->raybox[1]; a.bminy = g->raybox[2]; a.bmaxy = g->raybox[3];
-    a.L = g->raybox[5] - g->raybox[4];
-    a.pad = g->ray_pad; a.off = g->ray_offset; a.teps = g->t_eps;
-    a.flags = d_flags;
-    EventPair ev;
-    VCHECK(hipEventCreate(&ev.e0));
-    VCHECK(hipEventCreate(&ev.e1));
-    hipEvent_t e0 = ev.e0, e1 = ev.e1;
-    VCHECK(hipEventRecord(e0, st));
-    const size_t lds_bits = sizeof(uint32_t) * 64 * (size_t)((nz + 31) / 32);
-    hipLaunchKernelGGL(k_voxelize, dim3((unsigned)ntiles), dim3(64 * VWAVES), lds_bits, st, a);
-    hipError_t le = hipGetLastError();
-    VCHECK(hipEventRecord(e1, st));
-    VCHECK(le);
+    int32_t* d_list = nullptr;
+    auto list_buf = [&](int64_t n) -> int32_t* {
+        if (m.alloc(&d_list, (size_t)std::max<int64_t>(n, 1)) != hipSuccess) {
+            lbm_ctx_fail(ctx, LBM_ENOMEM, "lbm_voxelize: tile lists");
+            return nullptr;
+        }
+        return d_list;
+    };
+    int rc = bin_and_voxelize(ctx, "lbm_voxelize", d_tri, ntri, g, d_flags, b, list_buf, kernel_ms);
+    if (rc) return rc;
     VCHECK(hipMemcpyAsync(flags_out, d_flags, (size_t)nsites, hipMemcpyDeviceToHost, st));
     VCHECK(hipStreamSynchronize(st));
-    float ms = 0.f;
-    VCHECK(hipEventElapsedTime(&ms, e0, e1));
-    if (kernel_ms) *kernel_ms = ms;
     return LBM_OK;
 }
 

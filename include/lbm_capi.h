@@ -94,6 +94,21 @@ int lbm_grid_from_bbox(const double* tri, int64_t ntri, double spacing, lbm_grid
 int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const lbm_grid_desc* grid,
                  uint8_t* flags_out, double* kernel_ms);
 
+/* lbm_voxelize with device buffers, allocating nothing: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture of its own work, tight serving loops, harnesses that forbid
+ * hipMalloc).  tri_dev (12 doubles per facet) and flags_dev (one byte per grid
+ * point, x fastest) are device pointers on the context's device; the facet
+ * bins live in a fixed 64 MiB array per device that the library holds until
+ * the process ends.  A case whose bins exceed it (tile arrays of 12 bytes per
+ * 8 x 8 column tile, 20 bytes per facet, 4 bytes per facet-tile entry) fails
+ * with LBM_ERANGE before any device work.  Calls on one device share that
+ * array: calls that overlap on one device race, so serialise them.  Returns
+ * when the flags are written (one 32-byte device-to-host read of the bin
+ * counts in between).  Same flags as lbm_voxelize, bit for bit. */
+int lbm_voxelize_dev(lbm_ctx* ctx, const double* tri_dev, int64_t ntri, const lbm_grid_desc* grid,
+                     uint8_t* flags_dev, double* kernel_ms);
+
 typedef struct {
     int32_t axis;      /* coordinate tested against the plane value */
     int32_t nranges;   /* 0..3 extra open-interval conditions */

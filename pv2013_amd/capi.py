@@ -33,7 +33,7 @@ EXPORTS = (
     "lbm_domain_destroy", "lbm_domain_info", "lbm_set_waveform", "lbm_step", "lbm_step_timed",
     "lbm_sync", "lbm_iteration", "lbm_get_f", "lbm_set_f", "lbm_get_macros", "lbm_get_wss",
     "lbm_reduce", "lbm_get_macros_box", "lbm_get_wss_box", "lbm_batch_create", "lbm_autotune",
-    "lbm_domain_get_stats", "lbm_ctx_bandwidth",
+    "lbm_domain_get_stats", "lbm_ctx_bandwidth", "lbm_voxelize_dev",
 )
 
 
@@ -108,6 +108,7 @@ def lib():
         L.lbm_read_stl_binary.restype = i64
         L.lbm_grid_from_bbox.argtypes = [vp, i64, dbl, P(GridDesc)]
         L.lbm_voxelize.argtypes = [vp, vp, i64, P(GridDesc), vp, P(dbl)]
+        L.lbm_voxelize_dev.argtypes = [vp, vp, i64, P(GridDesc), vp, P(dbl)]
         L.lbm_classify.argtypes = [vp, P(GridDesc), vp, i32, vp, P(dbl)]
         L.lbm_domain_create.argtypes = [vp, P(DomainDesc), P(vp)]
         L.lbm_domain_destroy.argtypes = [vp]

@@ -51,6 +51,8 @@ int lbm_ctx_device(lbm_ctx* ctx) { return ctx->device; }
 
 extern "C" int lbm_version(void) { return 1; }
 
+int lbm_vox_scratch_init(int device);  // voxelize.hip
+
 extern "C" int lbm_ctx_create(int device, lbm_ctx** out) {
     if (!out) return LBM_EINVAL;
     *out = nullptr;
@@ -58,6 +60,8 @@ extern "C" int lbm_ctx_create(int device, lbm_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return LBM_ENODEV;
     if (device < 0 || device >= n) return LBM_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return LBM_ENODEV;
+    // the device address of lbm_voxelize_dev's scratch array, with the device current
+    if (lbm_vox_scratch_init(device) != LBM_OK) return LBM_EIO;
     lbm_ctx* c = new lbm_ctx;
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||

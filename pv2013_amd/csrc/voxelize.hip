@@ -30,6 +30,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "lbm_internal.h"
@@ -489,7 +490,28 @@ int bin_and_voxelize(lbm_ctx* ctx, const char* fn, const double* d_tri, int64_t 
     return LBM_OK;
 }
 
+// The no-allocation entry point's binning buffers: one array per device the
+// library runs on, held until the process ends.  Host code uses only the
+// device address hipGetSymbolAddress gives for it on each device.
+constexpr size_t VOX_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int VOX_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[VOX_SCRATCH_BYTES];
+void* g_scratch_ptr[VOX_MAX_DEVICES];
+std::mutex g_scratch_mu;
+
 }  // namespace
+
+// Called by lbm_ctx_create with `device` current: the address of this
+// device's scratch array, once per device.
+int lbm_vox_scratch_init(int device) {
+    if (device < 0 || device >= VOX_MAX_DEVICES) return LBM_ERANGE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return LBM_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LBM_EIO;
+    g_scratch_ptr[device] = p;
+    return LBM_OK;
+}
 
 extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const lbm_grid_desc* g,
                             uint8_t* flags_out, double* kernel_ms) {
@@ -525,6 +547,40 @@ extern "C" int lbm_voxelize(lbm_ctx* ctx, const double* tri, int64_t ntri, const
     VCHECK(hipMemcpyAsync(flags_out, d_flags, (size_t)nsites, hipMemcpyDeviceToHost, st));
     VCHECK(hipStreamSynchronize(st));
     return LBM_OK;
+}
+
+extern "C" int lbm_voxelize_dev(lbm_ctx* ctx, const double* tri_dev, int64_t ntri, const lbm_grid_desc* g,
+                                uint8_t* flags_dev, double* kernel_ms) {
+    if (!ctx) return LBM_EINVAL;
+    if (!g || !flags_dev || (ntri > 0 && !tri_dev) || ntri < 0)
+        return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_voxelize_dev: null argument");
+    if (int rc = check_grid(ctx, "lbm_voxelize_dev", ntri, g)) return rc;
+    const int dev = lbm_ctx_device(ctx);
+    VCHECK(hipSetDevice(dev));
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = dev >= 0 && dev < VOX_MAX_DEVICES ? g_scratch_ptr[dev] : nullptr;
+    }
+    if (!base) return lbm_ctx_fail(ctx, LBM_EIO, "lbm_voxelize_dev: no scratch address for this device");
+    const int nx = g->dims[0], ny = g->dims[1];
+    const int64_t ntiles = (int64_t)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY);
+    const size_t fixed = bin_layout(ntri, ntiles, nullptr, nullptr);
+    if (fixed > VOX_SCRATCH_BYTES)
+        return lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize_dev: " + std::to_string(fixed) +
+                                                 " bytes of bins exceed the 64 MiB scratch");
+    BinBufs b;
+    bin_layout(ntri, ntiles, (char*)base, &b);
+    const int64_t cap = (int64_t)((VOX_SCRATCH_BYTES - fixed) / sizeof(int32_t));
+    auto list_buf = [&](int64_t n) -> int32_t* {
+        if (n > cap) {
+            lbm_ctx_fail(ctx, LBM_ERANGE, "lbm_voxelize_dev: " + std::to_string(n) +
+                                              " tile-list entries exceed the 64 MiB scratch");
+            return nullptr;
+        }
+        return (int32_t*)((char*)base + fixed);
+    };
+    return bin_and_voxelize(ctx, "lbm_voxelize_dev", tri_dev, ntri, g, flags_dev, b, list_buf, kernel_ms);
 }
 
 // Index interval of grid points along one axis that satisfy a plane's

@@ -286,31 +286,91 @@ __global__ __launch_bounds__(256) void k_bin_zero(uint32_t* p, int64_t n) {
     if (i < n) p[i] = 0u;
 }
 
+// Tile range of facet t (x0 > x1: none); slivers go to the every-column list.
+__device__ __forceinline__ int4 facet_tiles(const BinArgs& a, int64_t t, unsigned long long* nglob_slot) {
+    const double* T = a.tri + 12 * t;
+    int4 r = make_int4(1, 0, 1, 0);
+    if (T[11] == 0.0) return r;
+    if (thin_xy(T)) {
+        const unsigned long long pos = atomicAdd(nglob_slot, 1ull);
+        a.b.glob[pos] = (int32_t)t;
+        return r;
+    }
+    const double xlo = fmin(T[0], fmin(T[3], T[6])) - a.slack, xhi = fmax(T[0], fmax(T[3], T[6])) + a.slack;
+    const double ylo = fmin(T[1], fmin(T[4], T[7])) - a.slack, yhi = fmax(T[1], fmax(T[4], T[7])) + a.slack;
+    const double i0 = fmax(floor((xlo - a.ox) / a.sp) - 1.0, 0.0);
+    const double i1 = fmin(ceil((xhi - a.ox) / a.sp) + 1.0, (double)(a.nx - 1));
+    const double j0 = fmax(floor((ylo - a.oy) / a.sp) - 1.0, 0.0);
+    const double j1 = fmin(ceil((yhi - a.oy) / a.sp) + 1.0, (double)(a.ny - 1));
+    if (i0 <= i1 && j0 <= j1) r = make_int4((int)i0 / TX, (int)i1 / TX, (int)j0 / TY, (int)j1 / TY);
+    return r;
+}
+
 __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.ntri) return;
-    const double* T = a.tri + 12 * t;
-    int4 r = make_int4(1, 0, 1, 0);
-    if (T[11] != 0.0) {
-        if (thin_xy(T)) {
-            const unsigned long long pos = atomicAdd(&a.b.hdr[1], 1ull);
-            a.b.glob[pos] = (int32_t)t;
-        } else {
-            const double xlo = fmin(T[0], fmin(T[3], T[6])) - a.slack, xhi = fmax(T[0], fmax(T[3], T[6])) + a.slack;
-            const double ylo = fmin(T[1], fmin(T[4], T[7])) - a.slack, yhi = fmax(T[1], fmax(T[4], T[7])) + a.slack;
-            const double i0 = fmax(floor((xlo - a.ox) / a.sp) - 1.0, 0.0);
-            const double i1 = fmin(ceil((xhi - a.ox) / a.sp) + 1.0, (double)(a.nx - 1));
-            const double j0 = fmax(floor((ylo - a.oy) / a.sp) - 1.0, 0.0);
-            const double j1 = fmin(ceil((yhi - a.oy) / a.sp) + 1.0, (double)(a.ny - 1));
-            if (i0 <= i1 && j0 <= j1) {
-                r = make_int4((int)i0 / TX, (int)i1 / TX, (int)j0 / TY, (int)j1 / TY);
-                for (int ty = r.z; ty <= r.w; ++ty)
-                    for (int tx = r.x; tx <= r.y; ++tx) atomicAdd(&a.b.cnt[(int64_t)ty * a.ntx + tx], 1);
-                atomicAdd(&a.b.hdr[0], (unsigned long long)(r.y - r.x + 1) * (unsigned long long)(r.w - r.z + 1));
-            }
-        }
-    }
+    const int4 r = facet_tiles(a, t, &a.b.hdr[1]);
     a.b.rng[t] = r;
+    if (r.x > r.y) return;
+    for (int ty = r.z; ty <= r.w; ++ty)
+        for (int tx = r.x; tx <= r.y; ++tx) atomicAdd(&a.b.cnt[(int64_t)ty * a.ntx + tx], 1);
+    atomicAdd(&a.b.hdr[0], (unsigned long long)(r.y - r.x + 1) * (unsigned long long)(r.w - r.z + 1));
+}
+
+// The same two passes with the tile histogram privatised in LDS, for grids of
+// at most LDS_TILES tiles: a workgroup takes FPW consecutive facets (meshes
+// list neighbouring facets together, so a workgroup touches few tiles), counts
+// into LDS, then adds each non-zero count to the global count with one atomic;
+// the fill pass reserves one block of positions per tile per workgroup and
+// hands out positions inside it with LDS atomics.
+constexpr int LDS_TILES = 16384;  // 64 KiB of LDS counters
+constexpr int FPW = 2048;         // facets per workgroup
+
+__global__ __launch_bounds__(256) void k_bin_count_lds(BinArgs a) {
+    extern __shared__ int32_t hist[];  // ntiles counters
+    const int n = (int)a.ntiles;
+    for (int i = threadIdx.x; i < n; i += 256) hist[i] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * FPW;
+    const int64_t t1 = t0 + FPW < a.ntri ? t0 + FPW : a.ntri;
+    unsigned long long entries = 0;
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        const int4 r = facet_tiles(a, t, &a.b.hdr[1]);
+        a.b.rng[t] = r;
+        for (int ty = r.z; ty <= r.w; ++ty)
+            for (int tx = r.x; tx <= r.y; ++tx) atomicAdd(&hist[ty * a.ntx + tx], 1);
+        if (r.x <= r.y) entries += (unsigned long long)(r.y - r.x + 1) * (unsigned long long)(r.w - r.z + 1);
+    }
+    if (entries) atomicAdd(&a.b.hdr[0], entries);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256)
+        if (hist[i]) atomicAdd(&a.b.cnt[i], hist[i]);
+}
+
+__global__ __launch_bounds__(256) void k_bin_fill_lds(BinArgs a) {
+    extern __shared__ int32_t pos[];  // per tile: this workgroup's count, then its next position
+    const int n = (int)a.ntiles;
+    for (int i = threadIdx.x; i < n; i += 256) pos[i] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * FPW;
+    const int64_t t1 = t0 + FPW < a.ntri ? t0 + FPW : a.ntri;
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        const int4 r = a.b.rng[t];
+        for (int ty = r.z; ty <= r.w; ++ty)
+            for (int tx = r.x; tx <= r.y; ++tx) atomicAdd(&pos[ty * a.ntx + tx], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256)
+        if (pos[i]) pos[i] = atomicAdd(&a.b.cnt[i], pos[i]);  // the cursor: reserve a block
+    __syncthreads();
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        const int4 r = a.b.rng[t];
+        for (int ty = r.z; ty <= r.w; ++ty)
+            for (int tx = r.x; tx <= r.y; ++tx) {
+                const int32_t p = atomicAdd(&pos[ty * a.ntx + tx], 1);
+                if (p < a.b.list_cap) a.b.list[p] = (int32_t)t;
+            }
+    }
 }
 
 __device__ __forceinline__ int length_bucket(int len) { return len <= 0 ? 0 : 32 - __clz(len); }
@@ -451,9 +511,13 @@ int bin_and_voxelize(lbm_ctx* ctx, const char* fn, const double* d_tri, int64_t 
     const int64_t zw = (int64_t)((char*)b.off - (char*)b.hdr) / 4;  // hdr and cnt
     hipLaunchKernelGGL(k_bin_zero, dim3((unsigned)((zw + 255) / 256)), dim3(256), 0, st, (uint32_t*)b.hdr, zw);
     VCHECK(hipGetLastError());
+    const bool lds = ntiles <= LDS_TILES;
     const unsigned fgrid = (unsigned)std::max<int64_t>(1, (ntri + 255) / 256);
+    const unsigned lgrid = (unsigned)std::max<int64_t>(1, (ntri + FPW - 1) / FPW);
+    const size_t lds_hist = sizeof(int32_t) * (size_t)ntiles;
     if (ntri > 0) {
-        hipLaunchKernelGGL(k_bin_count, dim3(fgrid), dim3(256), 0, st, ba);
+        if (lds) hipLaunchKernelGGL(k_bin_count_lds, dim3(lgrid), dim3(256), lds_hist, st, ba);
+        else hipLaunchKernelGGL(k_bin_count, dim3(fgrid), dim3(256), 0, st, ba);
         VCHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(SCAN_THREADS), 0, st, ba);
@@ -467,7 +531,8 @@ int bin_and_voxelize(lbm_ctx* ctx, const char* fn, const double* d_tri, int64_t 
     if (!ba.b.list) return LBM_ERANGE;
     ba.b.list_cap = (int64_t)hdr[0];
     if (ntri > 0) {
-        hipLaunchKernelGGL(k_bin_fill, dim3(fgrid), dim3(256), 0, st, ba);
+        if (lds) hipLaunchKernelGGL(k_bin_fill_lds, dim3(lgrid), dim3(256), lds_hist, st, ba);
+        else hipLaunchKernelGGL(k_bin_fill, dim3(fgrid), dim3(256), 0, st, ba);
         VCHECK(hipGetLastError());
     }
     VoxArgs a;

@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+INFINITY_CACHE_BYTES = 256 * 2 ** 20  # MI355X last-level (MALL) cache
 TAU = 0.66
 SEED = 2013
 PERIOD = 4000.0
@@ -99,13 +100,19 @@ def cpu_baseline(case, budget_s=10.0, max_threads=16):
 
 def traffic_from_profiles(key):
     """HBM bytes per step measured with rocprofv3 FETCH_SIZE / WRITE_SIZE for
-    this storage / scheme / layout (profiles/step_traffic.json), or None."""
+    this storage / scheme / layout, and where that figure comes from
+    (profiles/step_traffic.json: the profile file, command and commit), or
+    (None, None).  Counters cannot be read from inside the run, so the figure
+    is the profiled run's, not this run's."""
     path = os.path.join(ROOT, "profiles", "step_traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(key)
+            t = json.load(fh)
     except (OSError, ValueError):
-        return None
+        return None, None
+    if key not in t:
+        return None, None
+    return t[key], t.get("_source")
 
 
 def main():
@@ -163,6 +170,21 @@ def main():
     # size of the lattice (the AA scheme's access pattern) and a 1 GiB copy
     extra["measured_gbs"] = {"in_place_lattice_size": ctx.bandwidth(n * 19 * sbytes, in_place=True),
                              "copy_1GiB": ctx.bandwidth(1 << 30, in_place=False, reps=5)}
+    # The AA lattice of this case (19 x 4 B x 2.74M nodes = 209 MB) fits in the
+    # 256 MB Infinity Cache, so the headline step is not purely HBM-streaming.
+    # The same storage with two lattices (418 MB, larger than the cache) runs
+    # beside it as the HBM-streaming figure.
+    lattice_bytes = 19 * sbytes * r["stats"]["nodes"] * (1 if scheme == P.SCHEME_AA else 2)
+    streaming = None
+    if scheme == P.SCHEME_AA and not args.no_compare:
+        print("bench: HBM-streaming run (two lattices)", file=sys.stderr, flush=True)
+        rs = run(P, ctx, case, storage, min(args.steps, 1000), min(args.warmup, 100), scheme=P.SCHEME_TWO_LATTICE,
+                 options=options, block=args.block)
+        ms2 = rs["ms"] / min(args.steps, 1000)
+        ach2 = bytes_per_launch / (ms2 * 1e-3) / 1e9
+        streaming = {"scheme": "two-lattice pull", "lattice_bytes": 2 * 19 * sbytes * rs["nodes"],
+                     "cache_resident": False, "mlups": rs["nodes"] / (ms2 * 1e-3) / 1e6,
+                     "achieved": ach2, "frac": ach2 / HBM_PEAK_GBS}
     if not args.no_compare:
         for name, st in (("f64", P.STORE_F64), ("f16s", P.STORE_F16S)):
             if name == args.storage:
@@ -175,6 +197,7 @@ def main():
             extra["speedup_vs_f64"] = mlups / extra["mlups_f64"]
         print("bench: accuracy protocol (%d steps per storage)" % case["steps"], file=sys.stderr, flush=True)
         extra["errors_vs_f64"] = accuracy(P, ctx, case, (("f32", P.STORE_F32), ("f16s", P.STORE_F16S)))
+    traffic, traffic_src = traffic_from_profiles("%s_%s_%s" % (args.storage, args.scheme, args.layout))
     cpu = None
     if not args.no_cpu:
         print("bench: CPU oracle sample", file=sys.stderr, flush=True)
@@ -200,8 +223,11 @@ def main():
                    "scheme": "AA" if args.scheme == "aa" else "two-lattice pull",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles("%s_%s_%s" % (args.storage, args.scheme, args.layout))},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "lattice_bytes": lattice_bytes,
+                     "cache_resident": lattice_bytes <= INFINITY_CACHE_BYTES,
+                     "hbm_streaming": streaming},
         "cpu_baseline": cpu,
         "extra": extra,
     }

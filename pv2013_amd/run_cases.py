@@ -89,10 +89,10 @@ def run_c4(ctx, quick):
             "lane_use": res["f32"]["lane_use"]}
 
 
-def run_c5(ctx, quick):
+def run_c5(ctx, quick, steps=None):
     dims = (384, 192, 192) if quick else (768, 384, 384)
-    steps = 1000 if quick else 10000
-    cmp_steps = 500 if quick else 2000
+    steps = steps or (1000 if quick else 10000)
+    cmp_steps = min(500 if quick else 2000, steps // 2)
     t0 = time.perf_counter()
     c = cases.c5(ctx, seed=5, dims=dims, h=4.0 if quick else 3.0)
     gen = time.perf_counter() - t0
@@ -120,12 +120,17 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cases", nargs="+", choices=["c3", "c4", "c5"])
     ap.add_argument("--quick", action="store_true", help="reduced sizes for a smoke run")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="c5 only: total FP16 steps (default 10000; for short profiled runs)")
     args = ap.parse_args()
     ctx = P.Context(0)
     for name in args.cases:
         print("run_cases: %s" % name, file=sys.stderr, flush=True)
-        fn = {"c3": run_c3, "c4": run_c4, "c5": run_c5}[name]
-        print(json.dumps(fn(ctx, args.quick), default=float), flush=True)
+        if name == "c5":
+            res = run_c5(ctx, args.quick, args.steps)
+        else:
+            res = {"c3": run_c3, "c4": run_c4}[name](ctx, args.quick)
+        print(json.dumps(res, default=float), flush=True)
     ctx.close()
 
 

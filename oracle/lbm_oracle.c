@@ -30,13 +30,21 @@
  *                        build here (ComplexGeometry.cpp includes mpi_call.h,
  *                        whose template at :143 gcc and clang reject).
  *   orc_classify         ComplexGeometry.cpp:509-638, Subdomain.cpp:375-478.
- *                        Pinned by the PV2013 inlet/outlet/wall fixtures.
+ *                        Pinned by the PV2013 inlet/outlet/wall fixtures, and,
+ *                        through orc_run, by the reference's eight
+ *                        boundary-point files byte for byte (SHA-256 digests in
+ *                        tests/golden, tests/test_oracle_driver.py).
  *   collision            Simulation.cpp:21-179.  Pinned by: rest state is a fixed
  *                        point, moments of f_eq equal (rho, rho u), BGK relaxes
- *                        the non-equilibrium part by (1 - omega).
+ *                        the non-equilibrium part by (1 - omega)
+ *                        (tests/test_oracle_lbm.py, test_oracle_bc.py), and by a
+ *                        textbook BGK evaluation in numpy on every bounced link
+ *                        (tests/test_oracle_pins.py).
  *   streaming            Simulation.cpp:184-198, Node.cpp:16-20 (push with the
- *                        f != 0 guard).  Pinned by exact delta shifts on a
- *                        periodic-free box and mass conservation in a closed box.
+ *                        f != 0 guard).  Pinned by exact delta shifts with the
+ *                        collision switched off, mass conservation in a closed
+ *                        box, and the periodic shear wave decaying at the
+ *                        lattice viscosity (tests/test_oracle_lbm.py).
  *   Zou-He inlet         Boundary.cpp:127-148 (D3Q19), :85-123 (D3Q27).  Pinned by
  *                        the reference's Boundary::ApplyBc built from its sources
  *                        (oracle/_ref) on random inputs, and by the moment
@@ -45,22 +53,41 @@
  *                        rho = rho_out and zero tangential u after the update, and
  *                        for the x-face typo the closed-form residual
  *                        rho u_z = f16 - f10 that the f10-for-f16 swap leaves.
- *   bounce-back          Simulation.cpp:1353-1370, Node.cpp:121-125.  Pinned by a
- *                        resting closed box staying at rest and by Poiseuille flow
- *                        between plates (second-order wall location).
- *   WSS                  Simulation.cpp:1374-1421.  Pinned by a linear shear flow
- *                        with a closed-form gradient per neighbour set.
+ *   bounce-back          Simulation.cpp:1353-1370, Node.cpp:121-125.  Pinned by
+ *                        one step from a random state: every missing link d of a
+ *                        wall node puts the node's own BGK f_new[d] into
+ *                        f[opp d]; with the reference's loop from d = 1 (D3Q19)
+ *                        f[1] of a node whose +x neighbour is missing keeps its
+ *                        pre-step value bit for bit (tests/test_oracle_pins.py);
+ *                        and by a resting closed box staying at rest and
+ *                        Poiseuille flow (tests/test_oracle_lbm.py).
+ *   stale links          Inlet and outlet nodes get no bounce-back
+ *                        (WallBoundaryNode_ only): a population whose source is
+ *                        missing and that Zou-He does not rewrite keeps its
+ *                        pre-step value bit for bit (tests/test_oracle_pins.py).
+ *   WSS                  Simulation.cpp:1374-1421.  Pinned by a linear velocity
+ *                        field u = u0 + G x at equilibrium: mu |G m| at every
+ *                        wall node, m the mean of c_d over the links whose node
+ *                        exists, the rest link included, computed from the box
+ *                        geometry in numpy (tests/test_oracle_pins.py).
  *   waveform             Simulation.cpp:875-900, pulsatile.h:12-28 (PV2013 mode);
- *                        the Fourier mode has no reference counterpart.  Pinned
- *                        against long-double evaluation.
+ *                        the Fourier mode has no reference counterpart.  Both
+ *                        pinned against long-double evaluation within a rounding
+ *                        bound, with the zero gap in [0.81, 0.9) s, the value
+ *                        0.1423 at iteration 0 and the period
+ *                        (tests/test_oracle_pins.py).
  *   orc_sim_step         Simulation.cpp:903-932: Collision, Streaming, inlet,
  *                        wall, outlets, in that order, over the node lists.  Each
- *                        part is pinned as above, and the whole by mass
- *                        conservation, the resting box and Poiseuille flow.  The
- *                        reference's Simulation.cpp does not build here (it needs
- *                        the VTK library through OutputFile.h), so the time
- *                        evolution of the quirk-laden step as a whole is parity
- *                        unpinned beyond those checks.
+ *                        part is pinned as above and every node belongs to one
+ *                        list, so the list order cannot change a value; the whole
+ *                        is pinned by mass conservation, the resting box, the
+ *                        shear wave and config 1's Poiseuille profile with the
+ *                        Hagen-Poiseuille pressure gradient (clean bounce-back).
+ *                        No independent multi-step run of the reference's own
+ *                        Simulation.cpp exists here (it needs the VTK library
+ *                        through OutputFile.h), so many steps of the quirk-laden
+ *                        step on a real geometry are pinned only through these
+ *                        per-step and closed-form checks.
  */
 #include "lbm_oracle.h"
 

@@ -119,7 +119,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
+    # At rest the stored deviations f - w are exact zeros, and a lattice of
+    # zeros streams about 15% faster than one holding a flow; the inlet's
+    # pressure wave needs about 900 steps to cross the 512-node vessel, so the
+    # default warm-up lets the flow fill the lattice before the clock starts.
+    ap.add_argument("--warmup", type=int, default=1500)
     ap.add_argument("--storage", default="f32", choices=["f32", "f64", "f16s"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-compare", action="store_true",
@@ -178,7 +182,7 @@ def main():
     streaming = None
     if scheme == P.SCHEME_AA and not args.no_compare:
         print("bench: HBM-streaming run (two lattices)", file=sys.stderr, flush=True)
-        rs = run(P, ctx, case, storage, min(args.steps, 1000), min(args.warmup, 100), scheme=P.SCHEME_TWO_LATTICE,
+        rs = run(P, ctx, case, storage, min(args.steps, 1000), args.warmup, scheme=P.SCHEME_TWO_LATTICE,
                  options=options, block=args.block)
         ms2 = rs["ms"] / min(args.steps, 1000)
         ach2 = bytes_per_launch / (ms2 * 1e-3) / 1e9
@@ -190,7 +194,7 @@ def main():
             if name == args.storage:
                 continue
             print("bench: comparison run %s" % name, file=sys.stderr, flush=True)
-            rr = run(P, ctx, case, st, min(args.steps, 1000), min(args.warmup, 100), scheme=scheme, options=options,
+            rr = run(P, ctx, case, st, min(args.steps, 1000), args.warmup, scheme=scheme, options=options,
                      block=args.block)
             extra["mlups_" + name] = rr["nodes"] * min(args.steps, 1000) / (rr["ms"] * 1e-3) / 1e6
         if "mlups_f64" in extra:

@@ -100,7 +100,7 @@ extern "C" int lbm_ctx_bandwidth(lbm_ctx* ctx, int64_t bytes, int in_place, int 
         return lbm_ctx_fail(ctx, LBM_ENOMEM, "probe buffer");
     }
     const int64_t n = bytes / 16;
-    hipError_t e = hipMemsetAsync(a, 0, (size_t)bytes, ctx->stream);
+    hipError_t e = launch_stream_fill(a, n, ctx->stream);
     if (e == hipSuccess) e = launch_stream_probe(a, b, n, ctx->stream);  // warm-up
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
     for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stream_probe(a, b, n, ctx->stream);

@@ -25,6 +25,30 @@ hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Probe contents: a 32-bit integer hash of the element index, read as floats
+// in [1, 2), so every word is non-zero and the low mantissa bits vary.  The
+// memory system streams all-zero data measurably faster than data of a real
+// lattice, so a probe over zeros would overstate the rate.
+__global__ __launch_bounds__(256) void k_stream_fill(float4* a, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t h = (uint32_t)i * 0x9E3779B1u;
+        float v[4];
+        for (int k = 0; k < 4; ++k) {
+            h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12;
+            v[k] = __builtin_bit_cast(float, 0x3F800000u | (h >> 9));
+        }
+        a[i] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+hipError_t launch_stream_fill(float4* a, int64_t n, hipStream_t s) {
+    if (!a || n <= 0) return hipErrorInvalidValue;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_stream_fill, dim3((unsigned)blocks), dim3(256), 0, s, a, n);
+    return hipGetLastError();
+}
+
 // Streaming probe for the measured-bandwidth figure: grid-stride float4 pass,
 // in place (b == nullptr: a[i] += 1, so no store can be dropped) or a copy.
 __global__ __launch_bounds__(256) void k_stream_probe(float4* a, float4* b, int64_t n) {

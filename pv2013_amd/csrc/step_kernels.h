@@ -499,7 +499,7 @@ __device__ __forceinline__ void write_pre(const StepArgs& a, int64_t n, const ty
 // ------------------------------------------------------------------- kernel
 
 template <int Q, class P>
-__device__ __forceinline__ void apply_bc(const StepArgs& a, int64_t r, int type, typename P::calc_t* f);
+__device__ __forceinline__ void apply_bc(const StepArgs& a, int type, int64_t r, typename P::calc_t* f);
 
 // Bounce-back, stale populations and Zou-He of boundary record r on the
 // gathered populations f (miss: links whose source node does not exist).
@@ -510,6 +510,7 @@ __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64
     using T = typename P::calc_t;
     const typename P::store_t* gin = (const typename P::store_t*)a.gin;
     const uint32_t bb = a.bbb[r];
+    const int type = a.btype[r];
     if (SCH == SCH_AA_LOCAL) miss = a.bmiss[r];
     // bounce-back: this node's own post-collision g[opp d] (SCH_AA_LOCAL:
     // A[d][x] already holds it, kept there by the gather step)
@@ -528,22 +529,34 @@ __device__ __forceinline__ void apply_record(const StepArgs& a, int64_t r, int64
         for (int d = 0; d < Q; ++d)
             if ((miss & ~bb) >> d & 1u) f[d] = (T)stale[(int64_t)d * a.nrec + r];
     }
-    apply_bc<Q, P>(a, r, a.btype[r], f);
+    apply_bc<Q, P>(a, type, r, f);
 }
 
-// Zou-He of boundary record r when its type is a boundary condition.
+// Zou-He on the lanes whose record r carries a boundary condition (type >= 3,
+// condition type - 3).  The callers load type next to their other
+// per-record words, ahead of the population loads.  The condition is made
+// wave-uniform one condition at a time (a wavefront's boundary lanes nearly
+// always share one): its table entry and waveform value are then uniform
+// loads and the closure runs for one face orientation.  The inlet profile
+// factor is read here, by the boundary lanes only.
 template <int Q, class P>
-__device__ __forceinline__ void apply_bc(const StepArgs& a, int64_t r, int type, typename P::calc_t* f) {
+__device__ __forceinline__ void apply_bc(const StepArgs& a, int type, int64_t r, typename P::calc_t* f) {
     using T = typename P::calc_t;
-    if (type >= 3) {
-        const FaceTab& F = a.faces[type - 3];
-        double uv[3] = {0.0, 0.0, 0.0};
-        if (F.kind == LBM_BC_VELOCITY) {
-            const double w = F.waveform >= 0 ? a.wvals[F.waveform * LBM_WCHUNK + a.wk] : 1.0;
-            const double fac = a.bprof[r] * w;
-            uv[0] = F.u[0] * fac; uv[1] = F.u[1] * fac; uv[2] = F.u[2] * fac;
+    uint64_t todo = __builtin_amdgcn_ballot_w64(type >= 3);
+    while (todo) {
+        const int t0 = __builtin_amdgcn_readlane(type, (int)__builtin_ctzll(todo));
+        const bool mine = type == t0;
+        todo &= ~__builtin_amdgcn_ballot_w64(mine);
+        if (mine) {
+            const FaceTab& F = a.faces[t0 - 3];
+            double uv[3] = {0.0, 0.0, 0.0};
+            if (F.kind == LBM_BC_VELOCITY) {
+                const double w = F.waveform >= 0 ? a.wvals[F.waveform * LBM_WCHUNK + a.wk] : 1.0;
+                const double fac = a.bprof[r] * w;
+                uv[0] = F.u[0] * fac; uv[1] = F.u[1] * fac; uv[2] = F.u[2] * fac;
+            }
+            ZouHe<Q, T>::apply(F, uv, f);
         }
-        ZouHe<Q, T>::apply(F, uv, f);
     }
 }
 
@@ -703,6 +716,7 @@ __device__ __forceinline__ void record_node(const StepArgs& a, int64_t r) {
     const int64_t n = a.rec_node[r];
     const uint32_t bb = a.bbb[r];
     const uint32_t lmiss = SCH == SCH_AA_LOCAL ? a.bmiss[r] : 0u;
+    const int type = a.btype[r];
     T f[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
@@ -719,7 +733,7 @@ __device__ __forceinline__ void record_node(const StepArgs& a, int64_t r) {
         }
         f[d] = (T)*p;
     }
-    apply_bc<Q, P>(a, r, a.btype[r], f);
+    apply_bc<Q, P>(a, type, r, f);
     // AA gather: the node at x + c_d is the source of direction opp d
     emit<Q, P, MODE, SCH, false>(a, n, f, [&](int d) {
         return d == L::rest ? (int32_t)n : a.rsrc[(int64_t)L::opp(d) * R + r];
@@ -799,6 +813,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     const int64_t r = (int64_t)h[2] + __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
     const uint32_t bb = isrec ? a.bbb[r] : 0u;
     const uint32_t lmiss = (SCH == SCH_AA_LOCAL && isrec) ? a.bmiss[r] : 0u;
+    const int type = isrec ? (int)a.btype[r] : 0;
     const S* stale = (const S*)a.bstale;
     uint32_t pm = 0u;  // directions whose source node exists for this lane
     if constexpr (LAT) {
@@ -840,7 +855,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
         };
         if (anyst) gather(std::true_type{});
         else gather(std::false_type{});
-        if (isrec) apply_bc<Q, P>(a, r, a.btype[r], f);
+        apply_bc<Q, P>(a, type, r, f);
         emit<Q, P, MODE, SCH, true, false, true>(a, n, f, [&](int d) {
             if (d == L::rest) return (int32_t)n;
             return (pm >> L::opp(d)) & 1u ? hb[L::opp(d)] + lane : -1;
@@ -862,7 +877,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
         }
         f[d] = (T)*p;
     }
-    if (isrec) apply_bc<Q, P>(a, r, a.btype[r], f);
+    apply_bc<Q, P>(a, type, r, f);
     // AA gather: the node at x + c_d is the source of direction opp d
     emit<Q, P, MODE, SCH, false>(a, n, f, [&](int d) {
         if (d == L::rest) return (int32_t)n;

@@ -191,6 +191,12 @@ typedef struct {
  * arrays.  Automatic for FP16 lattices of 4 GiB or more; as an option it lets
  * tests compare the two record paths.  Results are identical. */
 #define LBM_OPT_WIDE_RECORDS 128
+/* Two-lattice row chunks one per wavefront.  By default the two-lattice pull's
+ * chunks (which end at aligned 64-node words) are taken two to a wavefront,
+ * each lane addressing through its own chunk, which fills the lanes of words
+ * cut by a row end.  AA chunks are always one per wavefront.  Results are
+ * identical. */
+#define LBM_OPT_ONE_CHUNK_PER_WAVE 256
 
 /* SubDomain(Geometry&) (Subdomain.cpp:3-33: CreateNodes, Connectivity,
  * Initialize, GetFluidHaloLat) + Simulation's constructor set-up
@@ -210,6 +216,7 @@ typedef struct {
     int32_t scheme;
     int32_t layout;                /* 0 generic kernel, 1 row chunks (one launch), 2 records + segments (two) */
     int32_t block;                 /* step-kernel threads per block in use */
+    int64_t step_waves;            /* wavefronts of one step launch (row chunks: chunk pairs and single chunks) */
 } lbm_domain_stats;
 int lbm_domain_get_stats(const lbm_domain* dom, lbm_domain_stats* out);
 

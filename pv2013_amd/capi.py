@@ -21,6 +21,7 @@ OPT_PERIODIC_X, OPT_PERIODIC_Y, OPT_PERIODIC_Z = 4, 8, 16
 OPT_GENERIC_KERNEL = 32  # one thread per node through the site map
 OPT_SPLIT_LAUNCH = 64  # records and row segments in two launches
 OPT_WIDE_RECORDS = 128  # FP16 chunks with records through 64-bit addresses (automatic >= 4 GiB)
+OPT_ONE_CHUNK_PER_WAVE = 256  # two-lattice row chunks one per wavefront instead of two-chunk windows
 BC_VELOCITY, BC_PRESSURE = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 WAVE_NONE, WAVE_FOURIER, WAVE_PV2013 = 0, 1, 2
@@ -71,7 +72,7 @@ class DomainStats(C.Structure):
     _fields_ = [("nodes", C.c_int64), ("boundary_records", C.c_int64), ("wall_nodes", C.c_int64),
                 ("bc_nodes", C.c_int64), ("waves", C.c_int64), ("regular_waves", C.c_int64),
                 ("segments", C.c_int64), ("device_bytes", C.c_int64), ("fast_axis", C.c_int32),
-                ("scheme", C.c_int32), ("layout", C.c_int32), ("block", C.c_int32)]
+                ("scheme", C.c_int32), ("layout", C.c_int32), ("block", C.c_int32), ("step_waves", C.c_int64)]
 
 
 class TuneResult(C.Structure):

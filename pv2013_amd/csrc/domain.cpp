@@ -144,6 +144,8 @@ struct lbm_domain {
     int64_t nseg = 0;
     int32_t* chunk = nullptr;  // fused layout: row chunks
     int64_t nchunk = 0;
+    int32_t* wave = nullptr;   // fused layout: wavefront table (null: one chunk per wavefront)
+    int64_t nwave = 0;
     std::vector<int32_t> wused;      // waveform slots referenced by a boundary condition
     int32_t* d_wused = nullptr;
     double* wvals = nullptr;         // [LBM_MAXWAVE][LBM_WCHUNK]
@@ -309,6 +311,8 @@ static void fill_args(const lbm_domain* D, StepArgs& a) {
     a.nseg = D->nseg;
     a.chunk = D->chunk;
     a.nchunk = D->nchunk;
+    a.wave = D->wave;
+    a.nwave = D->nwave;
     a.bprof = D->bprof;
     a.bstale = D->bstale;
     a.faces = D->d_faces;
@@ -583,7 +587,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     // last one's source; segments also end at multiples of 64 node ids so a
     // wavefront's stores cover whole cache lines.  Every run this accepts is
     // contiguous by construction, and the bounds are checked again below.
-    std::vector<int32_t> rsrc, segs, chunks;
+    std::vector<int32_t> rsrc, segs, chunks, waves;
     if (D->layout == LBM_LAYOUT_SPLIT) {
         rsrc.assign((size_t)q * std::max<int64_t>(D->nrec, 1), -1);
         for (int64_t r = 0; r < D->nrec; ++r) {
@@ -629,8 +633,12 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     // stopped may not start again).  So per direction the lanes with a source
     // form one interval [lo, hi) reading base + lane.  Two-lattice chunks also
     // end at 64-node words, so a full chunk's stores to its own nodes are
-    // aligned; AA chunks store to neighbour runs and may span two words.
+    // aligned, and unless LBM_OPT_ONE_CHUNK_PER_WAVE is set the chunks of each
+    // word are taken two to a wavefront (the wave table below), so a word cut
+    // by a row end still fills its lanes.  AA chunks store to neighbour runs
+    // and may span two words (fewer, fuller chunks, one per wavefront).
     const bool word_chunks = D->scheme == LBM_SCHEME_TWO_LATTICE;
+    const bool pair_chunks = word_chunks && !(d.options & LBM_OPT_ONE_CHUNK_PER_WAVE);
     if (D->layout == LBM_LAYOUT_FUSED) {
         const int CW = lbm_chunk_w(q);
         std::vector<int32_t> cbase(q, 0), clo(q, -1), chi(q, -1);
@@ -700,6 +708,22 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             m = start + len;
         }
         D->nchunk = (int64_t)(chunks.size() / CW);
+        // Wave table: the chunks of one 64-node word, in order, two to a
+        // wavefront (a word with k chunks takes ceil(k / 2) wavefronts).  The
+        // second chunk of a pair starts where the first ends.
+        if (pair_chunks) {
+            for (int64_t c = 0; c < D->nchunk;) {
+                const int32_t* h = &chunks[c * CW];
+                const bool two = c + 1 < D->nchunk && (h[CW] >> 6) == (h[0] >> 6);
+                if (two && (h[CW] != h[0] + h[1] || h[1] + h[CW + 1] > 64 || c >= LBM_WAVE_PAIR)) {
+                    delete D;
+                    return lbm_ctx_fail(ctx, LBM_EIO, "internal error: bad chunk pair");
+                }
+                waves.push_back((int32_t)c | (two ? LBM_WAVE_PAIR : 0));
+                c += two ? 2 : 1;
+            }
+            D->nwave = (int64_t)waves.size();
+        }
     }
     // device allocation
     int rc = LBM_OK;
@@ -733,6 +757,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
         DCHECK(D->alloc(&D->seg, segs.size()));
     }
     if (D->layout == LBM_LAYOUT_FUSED) DCHECK(D->alloc(&D->chunk, chunks.size()));
+    if (D->nwave > 0) DCHECK(D->alloc(&D->wave, waves.size()));
     uint8_t* stale_raw = nullptr;
     DCHECK(D->alloc(&stale_raw, (size_t)q * std::max<int64_t>(D->nrec, 1) * stale_sb));
     D->bstale = stale_raw;
@@ -798,6 +823,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     if (D->layout == LBM_LAYOUT_FUSED &&
         (rc = upload(D, D->chunk, chunks.data(), sizeof(int32_t) * chunks.size())))
         return fail(rc);
+    if (D->nwave > 0 && (rc = upload(D, D->wave, waves.data(), sizeof(int32_t) * waves.size()))) return fail(rc);
     if ((rc = upload(D, D->d_faces, D->faces.data(), sizeof(FaceTab) * D->faces.size()))) return fail(rc);
     if ((rc = upload(D, D->d_waves, D->waves.data(), sizeof(WaveTab) * D->waves.size()))) return fail(rc);
     if ((rc = upload(D, D->d_wused, D->wused.data(), sizeof(int32_t) * D->wused.size()))) return fail(rc);
@@ -849,6 +875,7 @@ extern "C" int lbm_domain_get_stats(const lbm_domain* D, lbm_domain_stats* o) {
     o->waves = (D->n + 63) / 64;
     o->regular_waves = D->nreg_waves;
     o->segments = D->layout == LBM_LAYOUT_FUSED ? D->nchunk : D->nseg;
+    o->step_waves = D->layout == LBM_LAYOUT_FUSED ? (D->nwave > 0 ? D->nwave : D->nchunk) : D->nseg;
     o->layout = D->layout;
     o->device_bytes = D->device_bytes;
     o->fast_axis = D->perm[0];

@@ -54,6 +54,11 @@ struct StepArgs {
     int64_t nseg;
     const int32_t* chunk;   // [c][lbm_chunk_w(q)] row chunks (fused layout)
     int64_t nchunk;
+    // Wavefront table of the fused layout (null: wavefront j takes chunk j).
+    // Entry j: chunk index c in bits 0-29; bit 30 set: the wavefront also takes
+    // chunk c + 1, the next piece of the same aligned 64-node window.
+    const int32_t* wave;
+    int64_t nwave;
     int32_t rb8, sb8;       // record / segment blocks per XCD (two-launch layout, set by the launcher)
     const double* bprof;    // per record: velocity profile factor
     const void* bstale;     // [e][r] stale populations (double, or float deviations)
@@ -112,6 +117,7 @@ constexpr int LBM_SEG_W = 32;
 //   lanes' source x - c_d is missing (bounce-back or stale value).
 // 36 ints (144 B) for D3Q19, 48 ints (192 B) for D3Q27.
 constexpr int lbm_chunk_w(int q) { return q == 19 ? 36 : 48; }
+constexpr int32_t LBM_WAVE_PAIR = 1 << 30;  // wave table: the entry's chunk and the next one
 static_assert(6 + 19 + 10 <= 36 && 6 + 27 + 14 <= 48, "chunk row fits");
 
 // Waveform values are tabulated once per chunk of LBM_WCHUNK consecutive

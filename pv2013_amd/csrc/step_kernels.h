@@ -777,29 +777,75 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
 // neighbour or its own slot) for one buffer load; stale values take a second
 // load only in chunks that have such lanes.  Otherwise the choice is among
 // 64-bit addresses, which the compiler turns into per-lane branches.
-template <int Q, class P, int MODE, int SCH, bool LAT = false>
-__device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lane) {
+//
+// TWO: the wavefront takes two consecutive chunks of one aligned 64-node window
+// (piece A on lanes [0, lenA), piece B on the lanes after it), so a window cut
+// by a row end still fills its lanes.  The two chunk rows are copied into the
+// wavefront's slot of LDS (srow, 2 CW ints) first, and each lane reads its
+// piece's base and interval per direction from there where it uses them,
+// indexed by its lane within the piece; the two record masks join into one over
+// the wavefront (piece B's records follow piece A's in node order).  The rows
+// stay out of scalar and vector registers, so the pair path does not raise the
+// register count of the single-chunk path it shares a kernel with.
+template <int Q, class P, int MODE, int SCH, bool LAT = false, bool TWO = false>
+__device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lane, int32_t* srow = nullptr) {
     using L = Lat<Q>;
     using S = typename P::store_t;
     using T = typename P::calc_t;
-    const int32_t* h = a.chunk + c * lbm_chunk_w(Q);  // wave-uniform
-    const int32_t start = h[0], len = h[1];
+    constexpr int CW = lbm_chunk_w(Q);
+    static_assert(2 * CW > 64 && 2 * CW <= 128, "two chunk rows take two LDS words per lane at most");
+    const int32_t* h = a.chunk + c * CW;  // wave-uniform; piece B's row follows at h + CW
+    if constexpr (TWO) {
+        // every lane is still active here
+        srow[lane] = h[lane];
+        if (lane + 64 < 2 * CW) srow[lane + 64] = h[lane + 64];
+        __builtin_amdgcn_wave_barrier();
+    }
+    const int32_t start = h[0], lenA = h[1];
+    const int32_t len = TWO ? lenA + h[CW + 1] : lenA;
     if (lane >= len) return;
     const S* gin = (const S*)a.gin;
     const int64_t NS = a.ns;
     const int64_t n = (int64_t)start + lane;
-    const uint32_t mlo = (uint32_t)h[4], mhi = (uint32_t)h[5];
-    // The whole chunk row, read once into scalar registers before any store:
-    // a table read after a store could not be proved unclobbered and would
-    // become a per-lane vector load with a wait of its own.
-    int32_t hb[Q], hl[Q];
-#pragma unroll
-    for (int d = 0; d < Q; ++d) {
-        hb[d] = h[6 + d];
-        hl[d] = (int32_t)(((uint32_t)h[6 + Q + (d >> 1)] >> (16 * (d & 1))) & 0xffffu);
+    uint32_t mlo = (uint32_t)h[4], mhi = (uint32_t)h[5];
+    const bool pb = TWO && lane >= lenA;     // lane of piece B
+    const int li = pb ? lane - lenA : lane;  // lane within its piece
+    const int32_t* lr = TWO ? srow + (pb ? CW : 0) : nullptr;  // this lane's piece row (LDS)
+    if constexpr (TWO) {
+        // lenA <= 63: piece B holds at least one node of the window
+        const uint64_t mb = (uint64_t)(uint32_t)h[CW + 5] << 32 | (uint32_t)h[CW + 4];
+        const uint64_t m = ((uint64_t)mhi << 32 | mlo) | (mb << lenA);
+        mlo = (uint32_t)m;
+        mhi = (uint32_t)(m >> 32);
     }
+    // One chunk: the whole row, read once into scalar registers before any
+    // store (a table read after a store could not be proved unclobbered and
+    // would become a per-lane vector load with a wait of its own).
+    int32_t hb[Q], hl[Q];
+    if constexpr (!TWO) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            hb[d] = h[6 + d];
+            hl[d] = (int32_t)(((uint32_t)h[6 + Q + (d >> 1)] >> (16 * (d & 1))) & 0xffffu);
+        }
+    }
+    // base and lo | hi << 8 of direction d for this lane's piece
+    auto base = [&](int d) -> int32_t {
+        if constexpr (TWO) return lr[6 + d];
+        else return hb[d];
+    };
+    auto ivl = [&](int d) -> uint32_t {
+        if constexpr (TWO) return ((uint32_t)lr[6 + Q + (d >> 1)] >> (16 * (d & 1))) & 0xffffu;
+        else return (uint32_t)hl[d];
+    };
+    // the same base at the stores, read again rather than kept live across the
+    // collision (volatile: not merged with the read at the loads)
+    auto base_late = [&](int d) -> int32_t {
+        if constexpr (TWO) return ((const volatile int32_t*)lr)[6 + d];
+        else return hb[d];
+    };
     T f[Q];
-    if ((mlo | mhi) == 0u) {
+    if (!TWO && (mlo | mhi) == 0u) {
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
             if (SCH == SCH_AA_LOCAL || d == L::rest) f[d] = ldp<P>(gin, d, NS, n);
@@ -823,9 +869,9 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
 #pragma unroll
         for (int d = 0; d < Q; ++d)
             if (d != L::rest) {
-                // lo <= lane < hi as one unsigned compare
-                const uint32_t lo = (uint32_t)hl[d] & 0xffu, hi = ((uint32_t)hl[d] >> 8) & 0xffu;
-                pm |= (uint32_t)((uint32_t)lane - lo < hi - lo) << d;
+                // lo <= li < hi as one unsigned compare
+                const uint32_t w = ivl(d), lo = w & 0xffu, hi = (w >> 8) & 0xffu;
+                pm |= (uint32_t)((uint32_t)li - lo < hi - lo) << d;
             }
         constexpr uint32_t nonrest = (uint32_t)((1ull << Q) - 1ull) & ~(1u << L::rest);
         const uint32_t stn = isrec ? (~pm & ~bb & nonrest) : 0u;  // links that take the stale value
@@ -838,7 +884,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
                     e = (uint32_t)d * nS + (uint32_t)n;
                 } else {
                     const uint32_t enb =
-                        (uint32_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * nS + (uint32_t)(hb[d] + lane);
+                        (uint32_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * nS + (uint32_t)(base(d) + li);
                     const uint32_t eown = (uint32_t)(SCH == SCH_TWO ? L::opp(d) : d) * nS + (uint32_t)n;
                     e = (pm >> d & 1u) ? enb : eown;
                 }
@@ -858,7 +904,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
         apply_bc<Q, P>(a, type, r, f);
         emit<Q, P, MODE, SCH, true, false, true>(a, n, f, [&](int d) {
             if (d == L::rest) return (int32_t)n;
-            return (pm >> L::opp(d)) & 1u ? hb[L::opp(d)] + lane : -1;
+            return (pm >> L::opp(d)) & 1u ? base_late(L::opp(d)) + li : -1;
         });
         return;
     }
@@ -869,9 +915,10 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
         if (SCH == SCH_AA_LOCAL || d == L::rest) {
             p = ((lmiss & ~bb) >> d & 1u) ? st : gin + (int64_t)d * NS + n;
         } else {
-            const bool pres = lane >= (hl[d] & 0xff) && lane < ((hl[d] >> 8) & 0xff);
+            const uint32_t w = ivl(d);
+            const bool pres = (uint32_t)li >= (w & 0xffu) && (uint32_t)li < ((w >> 8) & 0xffu);
             pm |= (uint32_t)pres << d;
-            const S* nb = gin + (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * NS + ((int64_t)hb[d] + lane);
+            const S* nb = gin + (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * NS + ((int64_t)base(d) + li);
             const S* own = gin + (int64_t)(SCH == SCH_TWO ? L::opp(d) : d) * NS + n;
             p = pres ? nb : ((bb >> d & 1u) ? own : st);
         }
@@ -881,7 +928,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     // AA gather: the node at x + c_d is the source of direction opp d
     emit<Q, P, MODE, SCH, false>(a, n, f, [&](int d) {
         if (d == L::rest) return (int32_t)n;
-        return (pm >> L::opp(d)) & 1u ? hb[L::opp(d)] + lane : -1;
+        return (pm >> L::opp(d)) & 1u ? base_late(L::opp(d)) + li : -1;
     });
 }
 
@@ -894,8 +941,28 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     if constexpr (PART == PART_FUSED) {
         const int64_t lb = logical_block();
         if (lb >= a.nblocks) return;
-        const int64_t c = __builtin_amdgcn_readfirstlane((int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-        if (c < a.nchunk) chunk_node<Q, P, MODE, SCH, LAT>(a, c, (int)(threadIdx.x & 63));
+        const int64_t j = __builtin_amdgcn_readfirstlane((int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+        const int lane = (int)(threadIdx.x & 63);
+        if constexpr (SCH == SCH_TWO) {
+            // Two-lattice pull (stores to the wavefront's own aligned nodes):
+            // wavefronts through the wave table, chunk pairs or single chunks.
+            // An AA gather's pair lanes would have to shift their stores as
+            // well as their loads, which measured slower than one chunk each.
+            if (a.wave) {
+                // LDS slot per wavefront for the rows of a chunk pair
+                __shared__ int32_t srow[8][2 * lbm_chunk_w(Q)];
+                if (j < a.nwave) {
+                    const int32_t e = a.wave[j];
+                    const int64_t c = e & (LBM_WAVE_PAIR - 1);
+                    if (e & LBM_WAVE_PAIR) chunk_node<Q, P, MODE, SCH, LAT, true>(a, c, lane, srow[threadIdx.x >> 6]);
+                    else chunk_node<Q, P, MODE, SCH, LAT, false>(a, c, lane);
+                }
+                return;
+            }
+        }
+        if (j < a.nchunk) {
+            chunk_node<Q, P, MODE, SCH, LAT>(a, j, lane);
+        }
         return;
     }
     const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
@@ -946,7 +1013,9 @@ inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
     if (layout == LBM_LAYOUT_SPLIT && ((a.nseg > 0 && !a.seg) || (a.nrec > 0 && (!a.rsrc || !a.rec_node))))
         return false;
     if (layout == LBM_LAYOUT_SPLIT && a.nseg + a.nrec <= 0) return false;
-    if (layout == LBM_LAYOUT_FUSED && (!a.chunk || a.nchunk <= 0)) return false;
+    if (layout == LBM_LAYOUT_FUSED && (!a.chunk || a.nchunk <= 0 || (a.wave && a.nwave <= 0))) return false;
+    // the wave table (chunk pairs) serves the two-lattice pull only
+    if (a.wave && sch != SCH_TWO) return false;
     // the fused layout runs the AA local step on the generic kernel
     if (layout == LBM_LAYOUT_FUSED && sch == SCH_AA_LOCAL && (!a.bbits || !a.bpre)) return false;
     if (a.nrec > 0 && (!a.btype || !a.bbb || !a.bprof || !a.bstale || !a.bmiss)) return false;
@@ -955,6 +1024,10 @@ inline bool step_args_ok(int mode, int sch, int layout, const StepArgs& a) {
     if (mode == LBM_MODE_PRE && sch != SCH_TWO) return false;
     return sch >= SCH_TWO && sch <= SCH_AA_LOCAL && layout >= LBM_LAYOUT_GENERIC && layout <= LBM_LAYOUT_SPLIT;
 }
+
+// Wavefronts of a fused-layout launch: one per wave-table entry (two-lattice
+// pull with chunk pairs) or one per chunk.
+inline int64_t fused_waves(const StepArgs& a, int sch) { return a.wave && sch == SCH_TWO ? a.nwave : a.nchunk; }
 
 template <int Q, class P, int MODE, int SCH>
 void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
@@ -973,12 +1046,12 @@ void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
             // descriptor while the lattice is below 4 GiB
             if (b.wide)
                 hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED, false>),
-                                   dim3(lbm_grid_blocks(b.nchunk, b.block / 64)), dim3(b.block), 0, s, b);
+                                   dim3(lbm_grid_blocks(fused_waves(b, SCH), b.block / 64)), dim3(b.block), 0, s, b);
             else
                 hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED, true>),
-                                   dim3(lbm_grid_blocks(b.nchunk, b.block / 64)), dim3(b.block), 0, s, b);
+                                   dim3(lbm_grid_blocks(fused_waves(b, SCH), b.block / 64)), dim3(b.block), 0, s, b);
         } else {
-            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(b.nchunk, b.block / 64)),
+            hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED>), dim3(lbm_grid_blocks(fused_waves(b, SCH), b.block / 64)),
                                dim3(b.block), 0, s, b);
         }
     } else {
@@ -1005,7 +1078,8 @@ hipError_t launch_step_t(int mode, int sch, int layout, const StepArgs& a, hipSt
     if (!step_args_ok(mode, sch, layout, a)) return hipErrorInvalidValue;
     StepArgs b = a;
     const int wpb = a.block / 64;  // wavefronts per block
-    b.nblocks = (int)(layout == LBM_LAYOUT_FUSED ? (a.nchunk + wpb - 1) / wpb : (a.n + a.block - 1) / a.block);
+    const int64_t fw = fused_waves(a, sch);  // wavefronts of the fused launch
+    b.nblocks = (int)(layout == LBM_LAYOUT_FUSED ? (fw + wpb - 1) / wpb : (a.n + a.block - 1) / a.block);
     b.rb8 = (int)(((a.nrec + a.block - 1) / a.block + 7) / 8);
     b.sb8 = (int)(((a.nseg + wpb - 1) / wpb + 7) / 8);
     if (mode == LBM_MODE_STEP) launch_sch<Q, P, LBM_MODE_STEP>(sch, layout, b, s);

@@ -68,8 +68,9 @@ def near_eq_state(oracle, q, flags, rng, amp=0.01):
     return f
 
 
-# step kernels: records + segments in one launch, in two, generic per-node kernel
-LAYOUTS = [0, 64, 32]
+# step kernels: row chunks in one launch (two-lattice: chunk pairs), records +
+# segments in two launches, generic per-node kernel, one row chunk per wavefront
+LAYOUTS = [0, 64, 32, 256]
 
 
 @pytest.mark.parametrize("q", [19, 27])
@@ -80,7 +81,12 @@ def test_fp64_bitwise_all_faces(P, oracle, ctx, q, quirks, options):
     cl, bcs = six_face_box(oracle, dims)
     print("six-face box q=%d quirks=%d options=%d" % (q, quirks, options), flush=True)
     dom, ref = both(P, oracle, ctx, q, dims, cl, 0.7, bcs, quirks=quirks, options=options)
-    assert dom.stats()["layout"] == {0: 1, 64: 2, 32: 0}[options]
+    st = dom.stats()
+    assert st["layout"] == {0: 1, 64: 2, 32: 0, 256: 1}[options]
+    if options == 0:  # the words cut by row ends hold chunk pairs
+        assert st["step_waves"] < st["segments"]
+    if options == 256:
+        assert st["step_waves"] == st["segments"]
     f0 = near_eq_state(oracle, q, cl, np.random.default_rng(q + quirks))
     dom.set_f(f0)
     ref.set_f(f0)
@@ -96,7 +102,7 @@ def test_fp64_bitwise_all_faces(P, oracle, ctx, q, quirks, options):
 
 
 @pytest.mark.parametrize("fast_axis,options", [(-1, 0), (0, 0), (1, 0), (-1, 1), (-1, 2), (-1, 32), (-1, 64),
-                                               (0, 32)])
+                                               (0, 32), (-1, 256)])
 def test_fp64_bitwise_pipe_config1(P, oracle, ctx, fast_axis, options):
     """D3Q19 FP64, 32x32x128 z-pipe of radius 12, tau 0.8, parabolic inlet
     u_max 0.05, outlet rho 1.0, 2000 steps from rest (the small plain case);
@@ -209,7 +215,8 @@ def test_step_timed_equals_step(P, oracle, ctx, scheme):
 def test_aa_equals_two_lattice(P, oracle, ctx, storage):
     """The AA scheme (one lattice, alternating gather / local steps) computes
     exactly what the two-lattice pull computes, in every precision; with the
-    graph path, the eager path, odd and even step counts."""
+    graph path, the eager path, odd and even step counts.  The two-lattice pull
+    with one row chunk per wavefront equals its default (chunk pairs)."""
     dims = (16, 16, 40)
     cl, bcs = pipe_case(oracle, dims, 6.0, 0.03, waveform=0)
     bc = [P.make_bc(**b) for b in bcs]
@@ -220,7 +227,8 @@ def test_aa_equals_two_lattice(P, oracle, ctx, storage):
     aa_generic = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA,
                           options=P.capi.OPT_GENERIC_KERNEL)
     aa_split = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA, options=P.capi.OPT_SPLIT_LAUNCH)
-    others = (aa, aa_eager, aa_generic, aa_split)
+    two_single = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, options=P.capi.OPT_ONE_CHUNK_PER_WAVE)
+    others = (aa, aa_eager, aa_generic, aa_split, two_single)
     for d in (two,) + others:
         d.set_waveform(0, P.WAVE_FOURIER, 200.0, a, phi)
     for n in (1, 70, 3, 130):

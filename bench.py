@@ -165,15 +165,21 @@ def main():
              "voxels_per_s": case["nsites"] / (case["vox_ms"] * 1e-3),
              "layout": args.layout, "block": r["stats"]["block"], "boundary_records": r["stats"]["boundary_records"],
              "row_chunks" if args.layout == "fused" else "segments": r["stats"]["segments"],
+             "step_waves": r["stats"]["step_waves"],
              "lane_use": ((n if args.layout == "fused" else n - r["stats"]["boundary_records"])
-                          / max(1, 64 * r["stats"]["segments"])),
+                          / max(1, 64 * r["stats"]["step_waves"])),
              "node_order_fast_axis": r["stats"]["fast_axis"]}
     if args.layout == "generic":
         extra["regular_wave_fraction"] = r["stats"]["regular_waves"] / max(1, r["stats"]["waves"])
     # measured device bandwidth next to the published peak: in place over the
-    # size of the lattice (the AA scheme's access pattern) and a 1 GiB copy
+    # size of the lattice (the AA scheme's access pattern), a 1 GiB copy, and
+    # the step's own arrays moved with aligned accesses and no arithmetic (the
+    # practical ceiling of this layout), in place and into a second lattice
+    probe_aa = ctx.bandwidth_lattice(19, sbytes, n, in_place=True)
+    probe_two = ctx.bandwidth_lattice(19, sbytes, n, in_place=False, reps=10)
     extra["measured_gbs"] = {"in_place_lattice_size": ctx.bandwidth(n * 19 * sbytes, in_place=True),
-                             "copy_1GiB": ctx.bandwidth(1 << 30, in_place=False, reps=5)}
+                             "copy_1GiB": ctx.bandwidth(1 << 30, in_place=False, reps=5),
+                             "lattice_probe_in_place": probe_aa, "lattice_probe_two_lattices": probe_two}
     # The AA lattice of this case (19 x 4 B x 2.74M nodes = 209 MB) fits in the
     # 256 MB Infinity Cache, so the headline step is not purely HBM-streaming.
     # The same storage with two lattices (418 MB, larger than the cache) runs
@@ -188,7 +194,8 @@ def main():
         ach2 = bytes_per_launch / (ms2 * 1e-3) / 1e9
         streaming = {"scheme": "two-lattice pull", "lattice_bytes": 2 * 19 * sbytes * rs["nodes"],
                      "cache_resident": False, "mlups": rs["nodes"] / (ms2 * 1e-3) / 1e6,
-                     "achieved": ach2, "frac": ach2 / HBM_PEAK_GBS}
+                     "achieved": ach2, "frac": ach2 / HBM_PEAK_GBS,
+                     "practical_ceiling": {"gbs": probe_two, "frac": ach2 / probe_two}}
     if not args.no_compare:
         for name, st in (("f64", P.STORE_F64), ("f16s", P.STORE_F16S)):
             if name == args.storage:
@@ -231,6 +238,12 @@ def main():
                      "traffic_source": traffic_src,
                      "lattice_bytes": lattice_bytes,
                      "cache_resident": lattice_bytes <= INFINITY_CACHE_BYTES,
+                     "practical_ceiling": {
+                         "probe": "lbm_ctx_bandwidth_lattice: the step's 19 arrays of this node count and "
+                                  "storage read then written with aligned accesses and no arithmetic, "
+                                  + ("in place" if scheme == P.SCHEME_AA else "into a second lattice"),
+                         "gbs": probe_aa if scheme == P.SCHEME_AA else probe_two,
+                         "frac": achieved / (probe_aa if scheme == P.SCHEME_AA else probe_two)},
                      "hbm_streaming": streaming},
         "cpu_baseline": cpu,
         "extra": extra,

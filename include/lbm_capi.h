@@ -60,6 +60,17 @@ const char* lbm_last_error(const lbm_ctx* ctx);
  * repeated `reps` times after one warm-up pass; *gbs = bytes read + written per
  * second, in GB/s, from HIP events.  No reference counterpart. */
 int lbm_ctx_bandwidth(lbm_ctx* ctx, int64_t bytes, int in_place, int reps, double* gbs);
+/* Practical ceiling of a step's memory traffic: `q` (19 or 27) population
+ * arrays of `nodes` elements of `elem_bytes` (2, 4 or 8) bytes each, stored as
+ * the step stores them (one array per direction, stride rounded up to 64
+ * nodes).  One thread per node reads its q elements, then writes q elements:
+ * in place into the arrays in reverse order (the AA local step's pattern) or
+ * into a second lattice (the two-lattice pull's pattern with aligned reads).
+ * No arithmetic beyond an integer add.  Repeated `reps` times after one warm-up
+ * pass; *gbs = bytes read + written per second, GB/s, from HIP events.  No
+ * reference counterpart. */
+int lbm_ctx_bandwidth_lattice(lbm_ctx* ctx, int q, int elem_bytes, int64_t nodes, int in_place, int reps,
+                              double* gbs);
 int lbm_version(void);
 
 /* ----------------------------------------------------------------- geometry */

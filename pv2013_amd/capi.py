@@ -34,7 +34,7 @@ EXPORTS = (
     "lbm_domain_destroy", "lbm_domain_info", "lbm_set_waveform", "lbm_step", "lbm_step_timed",
     "lbm_sync", "lbm_iteration", "lbm_get_f", "lbm_set_f", "lbm_get_macros", "lbm_get_wss",
     "lbm_reduce", "lbm_get_macros_box", "lbm_get_wss_box", "lbm_batch_create", "lbm_autotune",
-    "lbm_domain_get_stats", "lbm_ctx_bandwidth", "lbm_voxelize_dev",
+    "lbm_domain_get_stats", "lbm_ctx_bandwidth", "lbm_ctx_bandwidth_lattice", "lbm_voxelize_dev",
 )
 
 
@@ -131,6 +131,7 @@ def lib():
         L.lbm_domain_get_stats.argtypes = [vp, P(DomainStats)]
         L.lbm_autotune.argtypes = [vp, P(DomainDesc), i32, C.c_char_p, P(TuneResult)]
         L.lbm_ctx_bandwidth.argtypes = [vp, i64, i32, i32, P(dbl)]
+        L.lbm_ctx_bandwidth_lattice.argtypes = [vp, i32, i32, i64, i32, i32, P(dbl)]
         _lib = L
     return _lib
 
@@ -239,6 +240,15 @@ class Context:
         g = C.c_double()
         self.check(lib().lbm_ctx_bandwidth(self.h, int(nbytes) // 16 * 16, 1 if in_place else 0, reps, C.byref(g)),
                    "lbm_ctx_bandwidth")
+        return g.value
+
+    def bandwidth_lattice(self, q, elem_bytes, nodes, in_place=True, reps=20):
+        """Measured GB/s of a step's traffic with no arithmetic and aligned
+        accesses: q arrays of nodes elements, read then written, in place or
+        into a second lattice (lbm_ctx_bandwidth_lattice)."""
+        g = C.c_double()
+        self.check(lib().lbm_ctx_bandwidth_lattice(self.h, q, elem_bytes, int(nodes), 1 if in_place else 0, reps,
+                                                   C.byref(g)), "lbm_ctx_bandwidth_lattice")
         return g.value
 
     def voxelize(self, tri, grid):

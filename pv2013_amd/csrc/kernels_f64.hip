@@ -49,26 +49,60 @@ hipError_t launch_stream_fill(float4* a, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Streaming probe for the measured-bandwidth figure: grid-stride float4 pass,
-// in place (b == nullptr: a[i] += 1, so no store can be dropped) or a copy.
+// Streaming probe for the measured-bandwidth figure: one float4 per thread,
+// a grid that covers the whole buffer, in place (b == nullptr: a[i] += 1, so no
+// store can be dropped) or a copy.  A grid-stride loop over a capped grid
+// moves the same bytes markedly slower on this GPU (about 4.7 against 6.2 TB/s
+// for a 1 GiB copy), so it would understate what the memory delivers.
 __global__ __launch_bounds__(256) void k_stream_probe(float4* a, float4* b, int64_t n) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        float4 v = a[i];
-        if (b) {
-            b[i] = v;
-        } else {
-            v.x += 1.f; v.y += 1.f; v.z += 1.f; v.w += 1.f;
-            a[i] = v;
-        }
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float4 v = a[i];
+    if (b) {
+        b[i] = v;
+    } else {
+        v.x += 1.f; v.y += 1.f; v.z += 1.f; v.w += 1.f;
+        a[i] = v;
     }
 }
 
 hipError_t launch_stream_probe(float4* a, float4* b, int64_t n, hipStream_t s) {
-    if (!a || n <= 0) return hipErrorInvalidValue;
-    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
-    hipLaunchKernelGGL(k_stream_probe, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n);
+    if (!a || n <= 0 || (n + 255) / 256 > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_stream_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
     return hipGetLastError();
+}
+
+// Lattice-shaped probe for the practical ceiling of a step's traffic: Q arrays
+// of stride ns (SoA, as the step stores them), one thread per node reads its Q
+// elements and then writes Q elements, in place into the arrays in reverse
+// order (the AA local step's pattern) or into a second lattice (the two-lattice
+// pull's pattern with aligned reads).  Integer adds keep every store live.
+template <int Q, class E>
+__global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, int64_t ns) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    E v[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) v[d] = a[d * ns + i];
+    E* o = b ? b : a;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) o[(Q - 1 - d) * ns + i] = (E)(v[d] + 1);
+}
+
+template <class E>
+static hipError_t launch_lattice_probe_t(int q, void* a, void* b, int64_t n, int64_t ns, hipStream_t s) {
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (q == 19) hipLaunchKernelGGL((k_lattice_probe<19, E>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns);
+    else hipLaunchKernelGGL((k_lattice_probe<27, E>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns);
+    return hipGetLastError();
+}
+
+hipError_t launch_lattice_probe(int q, int elem_bytes, void* a, void* b, int64_t n, int64_t ns, hipStream_t s) {
+    if (!a || n <= 0 || ns < n || (q != 19 && q != 27) || (n + 255) / 256 > INT32_MAX) return hipErrorInvalidValue;
+    if (elem_bytes == 2) return launch_lattice_probe_t<uint16_t>(q, a, b, n, ns, s);
+    if (elem_bytes == 4) return launch_lattice_probe_t<uint32_t>(q, a, b, n, ns, s);
+    if (elem_bytes == 8) return launch_lattice_probe_t<uint64_t>(q, a, b, n, ns, s);
+    return hipErrorInvalidValue;
 }
 
 // One block per used waveform slot, one thread per iteration of the chunk.

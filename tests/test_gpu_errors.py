@@ -106,3 +106,20 @@ def test_voxelizer_degenerate_inputs(P, ctx):
     expect(P, lambda: ctx.voxelize(tri, big), -34, "nz above 2048")
     flags, _ = ctx.voxelize(tri, g)
     assert int(flags.sum()) == 0
+
+
+def test_bandwidth_probes(P, ctx):
+    """The measured-bandwidth probes: positive rates for valid buffers (the
+    lattice-shaped probe in every element size, in place and into a second
+    lattice, for a node count that is not a multiple of 64), errors for bad
+    arguments."""
+    assert ctx.bandwidth(1 << 20, in_place=True, reps=2) > 0
+    assert ctx.bandwidth(1 << 20, in_place=False, reps=2) > 0
+    for q in (19, 27):
+        for eb in (2, 4, 8):
+            for inplace in (True, False):
+                assert ctx.bandwidth_lattice(q, eb, 10007, in_place=inplace, reps=2) > 0
+    expect(P, lambda: ctx.bandwidth_lattice(15, 4, 1000), -22, "bad argument")
+    expect(P, lambda: ctx.bandwidth_lattice(19, 3, 1000), -22, "bad argument")
+    expect(P, lambda: ctx.bandwidth_lattice(19, 4, 0), -22, "bad argument")
+    assert ctx.bandwidth_lattice(19, 4, 4096, reps=1) > 0

@@ -63,13 +63,24 @@ int lbm_ctx_bandwidth(lbm_ctx* ctx, int64_t bytes, int in_place, int reps, doubl
 /* Practical ceiling of a step's memory traffic: `q` (19 or 27) population
  * arrays of `nodes` elements of `elem_bytes` (2, 4 or 8) bytes each, stored as
  * the step stores them (one array per direction, stride rounded up to 64
- * nodes).  One thread per node reads its q elements, then writes q elements:
- * in place into the arrays in reverse order (the AA local step's pattern) or
- * into a second lattice (the two-lattice pull's pattern with aligned reads).
- * No arithmetic beyond an integer add.  Repeated `reps` times after one warm-up
- * pass; *gbs = bytes read + written per second, GB/s, from HIP events.  No
- * reference counterpart. */
-int lbm_ctx_bandwidth_lattice(lbm_ctx* ctx, int q, int elem_bytes, int64_t nodes, int in_place, int reps,
+ * nodes).  One thread per node reads its q elements, then writes q elements,
+ * with no arithmetic beyond an integer add.  `pattern`:
+ *   LBM_PROBE_TWO (0)          into a second lattice, aligned (the two-lattice
+ *                              pull's stores);
+ *   LBM_PROBE_IN_PLACE (1)     in place, into the arrays in reverse order (the
+ *                              AA local step);
+ *   LBM_PROBE_TWO_SHIFTED (2)  into a second lattice, each read one node along,
+ *                              so every 64-lane read run starts off its cache
+ *                              line as a neighbour run does (the pull);
+ *   LBM_PROBE_IN_PLACE_SHIFTED (3)  in place, reads and writes one node along
+ *                              (the AA gather).
+ * Repeated `reps` times after one warm-up pass; *gbs = bytes read + written
+ * per second, GB/s, from HIP events.  No reference counterpart. */
+#define LBM_PROBE_TWO 0
+#define LBM_PROBE_IN_PLACE 1
+#define LBM_PROBE_TWO_SHIFTED 2
+#define LBM_PROBE_IN_PLACE_SHIFTED 3
+int lbm_ctx_bandwidth_lattice(lbm_ctx* ctx, int q, int elem_bytes, int64_t nodes, int pattern, int reps,
                               double* gbs);
 int lbm_version(void);
 

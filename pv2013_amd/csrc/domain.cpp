@@ -115,14 +115,17 @@ extern "C" int lbm_ctx_bandwidth(lbm_ctx* ctx, int64_t bytes, int in_place, int 
     return LBM_OK;
 }
 
-extern "C" int lbm_ctx_bandwidth_lattice(lbm_ctx* ctx, int q, int elem_bytes, int64_t nodes, int in_place, int reps,
+extern "C" int lbm_ctx_bandwidth_lattice(lbm_ctx* ctx, int q, int elem_bytes, int64_t nodes, int pattern, int reps,
                                          double* gbs) {
     if (!ctx) return LBM_EINVAL;
     if ((q != 19 && q != 27) || (elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8) || nodes <= 0 ||
-        nodes > ((int64_t)1 << 34) || reps <= 0 || !gbs)
+        nodes > ((int64_t)1 << 34) || pattern < LBM_PROBE_TWO || pattern > LBM_PROBE_IN_PLACE_SHIFTED || reps <= 0 ||
+        !gbs)
         return lbm_ctx_fail(ctx, LBM_EINVAL, "lbm_ctx_bandwidth_lattice: bad argument");
     CHECK(hipSetDevice(ctx->device));
-    const int64_t ns = (nodes + 63) / 64 * 64;
+    const bool in_place = pattern == LBM_PROBE_IN_PLACE || pattern == LBM_PROBE_IN_PLACE_SHIFTED;
+    const int shift = pattern >= LBM_PROBE_TWO_SHIFTED ? 1 : 0;
+    const int64_t ns = (nodes + shift + 63) / 64 * 64;
     const size_t bytes = (size_t)q * (size_t)ns * (size_t)elem_bytes;
     void *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, bytes) != hipSuccess) return lbm_ctx_fail(ctx, LBM_ENOMEM, "probe lattice");
@@ -133,9 +136,10 @@ extern "C" int lbm_ctx_bandwidth_lattice(lbm_ctx* ctx, int q, int elem_bytes, in
     // non-zero contents (the memory system moves zeros faster)
     hipError_t e = launch_stream_fill((float4*)a, (int64_t)(bytes / 16), ctx->stream);
     if (e == hipSuccess && b) e = launch_stream_fill((float4*)b, (int64_t)(bytes / 16), ctx->stream);
-    if (e == hipSuccess) e = launch_lattice_probe(q, elem_bytes, a, b, nodes, ns, ctx->stream);  // warm-up
+    if (e == hipSuccess) e = launch_lattice_probe(q, elem_bytes, shift, a, b, nodes, ns, ctx->stream);  // warm-up
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_lattice_probe(q, elem_bytes, a, b, nodes, ns, ctx->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r)
+        e = launch_lattice_probe(q, elem_bytes, shift, a, b, nodes, ns, ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
     float ms = 0.f;

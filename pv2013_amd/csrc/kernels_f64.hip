@@ -76,32 +76,43 @@ hipError_t launch_stream_probe(float4* a, float4* b, int64_t n, hipStream_t s) {
 // of stride ns (SoA, as the step stores them), one thread per node reads its Q
 // elements and then writes Q elements, in place into the arrays in reverse
 // order (the AA local step's pattern) or into a second lattice (the two-lattice
-// pull's pattern with aligned reads).  Integer adds keep every store live.
-template <int Q, class E>
+// pull's pattern).  SHIFT = 1 moves the reads (and, in place, the writes) one
+// node along, so every 64-lane run starts off its cache line as a neighbour
+// run does (the pull's reads; the AA gather's reads and writes).  Integer adds
+// keep every store live.
+template <int Q, class E, int SHIFT>
 __global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, int64_t ns) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    const int64_t j = i + SHIFT;  // < ns: the stride leaves room for the shift
     E v[Q];
 #pragma unroll
-    for (int d = 0; d < Q; ++d) v[d] = a[d * ns + i];
+    for (int d = 0; d < Q; ++d) v[d] = a[d * ns + j];
     E* o = b ? b : a;
+    const int64_t k = b ? i : j;
 #pragma unroll
-    for (int d = 0; d < Q; ++d) o[(Q - 1 - d) * ns + i] = (E)(v[d] + 1);
+    for (int d = 0; d < Q; ++d) o[(Q - 1 - d) * ns + k] = (E)(v[d] + 1);
 }
 
-template <class E>
+template <class E, int SHIFT>
 static hipError_t launch_lattice_probe_t(int q, void* a, void* b, int64_t n, int64_t ns, hipStream_t s) {
     const dim3 grid((unsigned)((n + 255) / 256));
-    if (q == 19) hipLaunchKernelGGL((k_lattice_probe<19, E>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns);
-    else hipLaunchKernelGGL((k_lattice_probe<27, E>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns);
+    if (q == 19) hipLaunchKernelGGL((k_lattice_probe<19, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns);
+    else hipLaunchKernelGGL((k_lattice_probe<27, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns);
     return hipGetLastError();
 }
 
-hipError_t launch_lattice_probe(int q, int elem_bytes, void* a, void* b, int64_t n, int64_t ns, hipStream_t s) {
-    if (!a || n <= 0 || ns < n || (q != 19 && q != 27) || (n + 255) / 256 > INT32_MAX) return hipErrorInvalidValue;
-    if (elem_bytes == 2) return launch_lattice_probe_t<uint16_t>(q, a, b, n, ns, s);
-    if (elem_bytes == 4) return launch_lattice_probe_t<uint32_t>(q, a, b, n, ns, s);
-    if (elem_bytes == 8) return launch_lattice_probe_t<uint64_t>(q, a, b, n, ns, s);
+hipError_t launch_lattice_probe(int q, int elem_bytes, int shift, void* a, void* b, int64_t n, int64_t ns,
+                                hipStream_t s) {
+    if (!a || n <= 0 || ns < n + shift || (shift != 0 && shift != 1) || (q != 19 && q != 27) ||
+        (n + 255) / 256 > INT32_MAX)
+        return hipErrorInvalidValue;
+    if (elem_bytes == 2) return shift ? launch_lattice_probe_t<uint16_t, 1>(q, a, b, n, ns, s)
+                                      : launch_lattice_probe_t<uint16_t, 0>(q, a, b, n, ns, s);
+    if (elem_bytes == 4) return shift ? launch_lattice_probe_t<uint32_t, 1>(q, a, b, n, ns, s)
+                                      : launch_lattice_probe_t<uint32_t, 0>(q, a, b, n, ns, s);
+    if (elem_bytes == 8) return shift ? launch_lattice_probe_t<uint64_t, 1>(q, a, b, n, ns, s)
+                                      : launch_lattice_probe_t<uint64_t, 0>(q, a, b, n, ns, s);
     return hipErrorInvalidValue;
 }
 

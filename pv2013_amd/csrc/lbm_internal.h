@@ -131,8 +131,10 @@ hipError_t launch_set_iter(int64_t* iter, int64_t value, hipStream_t s);
 hipError_t launch_stream_probe(float4* a, float4* b, int64_t n, hipStream_t s);
 hipError_t launch_stream_fill(float4* a, int64_t n, hipStream_t s);
 // lattice-shaped probe: q arrays of elem_bytes elements, stride ns; b == nullptr:
-// in place (reversed array order), else into lattice b
-hipError_t launch_lattice_probe(int q, int elem_bytes, void* a, void* b, int64_t n, int64_t ns, hipStream_t s);
+// in place (reversed array order), else into lattice b; shift 1: reads (in place:
+// reads and writes) one node along, off their cache lines (needs ns >= n + 1)
+hipError_t launch_lattice_probe(int q, int elem_bytes, int shift, void* a, void* b, int64_t n, int64_t ns,
+                                hipStream_t s);
 hipError_t launch_wave_chunk(const WaveTab* waves, const int32_t* used, int nused, const int64_t* iter,
                              int64_t it0, int it_abs, double* wvals, hipStream_t s);
 // sch: 0 two-lattice pull, 1 AA gather step, 2 AA local step

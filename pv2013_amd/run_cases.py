@@ -13,8 +13,11 @@ c4  64 seeded straight and bent pipes (lengths 64-512, radii 8-40, curvature
     into one domain (one launch per step, hipGraph chunks), 4000 steps:
     aggregate MLUPS and per-case velocity error against the same cases in FP64.
 c5  D3Q27 768x384x384 vessel tree (3 bifurcation levels, radii 60 -> 15), FP16
-    shifted storage with FP32 collision, 10000 steps: MLUPS; wall shear stress
+    shifted storage with FP32 collision, 10000 steps: MLUPS (AA, and the
+    two-lattice pull timed on the same developed flow); wall shear stress
     error against an FP64 run, both after 2000 steps.
+c4 and c5 also report the practical ceiling: the step's own arrays moved
+in the step's access pattern with no arithmetic (lbm_ctx_bandwidth_lattice).
 """
 import argparse
 import json
@@ -71,7 +74,7 @@ def run_c4(ctx, quick):
         per_case = [dom.macros_box(inf["origin"], inf["dims"])[1] for inf in info]
         res[name] = {"mlups": stats["nodes"] * (steps - 64) / (ms * 1e-3) / 1e6, "nodes": stats["nodes"],
                      "u": per_case,
-                     "lane_use": stats["nodes"] / max(1, 64 * stats["segments"])}
+                     "lane_use": stats["nodes"] / max(1, 64 * stats["step_waves"])}
         dom.close()
     errs = [cases.errors(u64, u32, c["flags"] > 0)["l2"]
             for u64, u32, c in zip(res["f64"]["u"], res["f32"]["u"], cs)]
@@ -81,12 +84,26 @@ def run_c4(ctx, quick):
     single.step(steps)
     same = bool(np.array_equal(single.macros()[1], res["f64"]["u"][0]))
     single.close()
+    # the step's own arrays moved with no arithmetic, the AA pattern (gather
+    # runs one node off their lines, local step aligned): the practical
+    # ceiling of this layout on this node count
+    ceil = aa_ceiling(ctx, 19, 4, res["f32"]["nodes"])
     return {"case": "c4", "cases": ncases, "steps": steps, "generate_s": gen, "fluid_nodes": res["f32"]["nodes"],
             "mlups_f32": res["f32"]["mlups"], "mlups_f64": res["f64"]["mlups"],
             "roofline_frac_f32": res["f32"]["mlups"] * 152 / 1e3 / HBM,
+            "practical_ceiling_gbs": ceil, "practical_ceiling_frac_f32": res["f32"]["mlups"] * 152 / 1e3 / ceil,
             "velocity_l2_vs_f64": {"max": max(errs), "median": float(np.median(errs))},
             "packed_f64_equals_single_case": same,
             "lane_use": res["f32"]["lane_use"]}
+
+
+def aa_ceiling(ctx, q, eb, nodes):
+    """GB/s of the AA step's traffic with no arithmetic: the mean time of the
+    shifted in-place pass (the gather's runs start one node off their lines)
+    and the aligned one (the local step)."""
+    a = ctx.bandwidth_lattice(q, eb, nodes, pattern=P.PROBE_IN_PLACE, reps=5)
+    b = ctx.bandwidth_lattice(q, eb, nodes, pattern=P.PROBE_IN_PLACE_SHIFTED, reps=5)
+    return 2.0 / (1.0 / a + 1.0 / b)
 
 
 def run_c5(ctx, quick, steps=None):
@@ -103,6 +120,14 @@ def run_c5(ctx, quick, steps=None):
     ms = dom.step_timed(steps - cmp_steps)
     mass, _ = dom.reduce()
     dom.close()
+    # the two-lattice pull (chunk pairs) on the same case, in the same
+    # developed-flow regime: warmed up for cmp_steps, then timed
+    two = cases.make_domain(ctx, c, q=27, storage=P.STORE_F16S, scheme=P.SCHEME_TWO_LATTICE)
+    st2 = two.stats()
+    two.step(cmp_steps)
+    n2 = min(steps - cmp_steps, 2000)
+    ms2 = two.step_timed(n2)
+    two.close()
     ref = cases.make_domain(ctx, c, q=27, storage=P.STORE_F64, scheme=P.SCHEME_AA)
     ref.step(cmp_steps)
     w64 = ref.wss()
@@ -110,8 +135,18 @@ def run_c5(ctx, quick, steps=None):
     ref.close()
     wall = c["flags"] == 2
     mlups = st["nodes"] * (steps - cmp_steps) / (ms * 1e-3) / 1e6
+    mlups2 = st2["nodes"] * n2 / (ms2 * 1e-3) / 1e6
+    # practical ceilings: the step's 27 FP16 arrays moved with no arithmetic,
+    # in the AA pattern and the pull's (reads one node off their lines)
+    ceil_aa = aa_ceiling(ctx, 27, 2, st["nodes"])
+    ceil_two = ctx.bandwidth_lattice(27, 2, st["nodes"], pattern=P.PROBE_TWO_SHIFTED, reps=5)
     return {"case": "c5", "box": list(dims), "facets": c["ntri"], "generate_s": gen, "fluid_nodes": st["nodes"],
             "steps": steps, "mlups_f16s": mlups, "roofline_frac_f16s": mlups * 108 / 1e3 / HBM,
+            "practical_ceiling_gbs": ceil_aa, "practical_ceiling_frac_f16s": mlups * 108 / 1e3 / ceil_aa,
+            "two_lattice": {"mlups_f16s": mlups2, "timed_steps": n2, "step_waves": st2["step_waves"],
+                            "lane_use": st2["nodes"] / (64.0 * st2["step_waves"]),
+                            "roofline_frac_f16s": mlups2 * 108 / 1e3 / HBM, "practical_ceiling_gbs": ceil_two,
+                            "practical_ceiling_frac_f16s": mlups2 * 108 / 1e3 / ceil_two},
             "mlups_f64": st["nodes"] * 200 / (ms64 * 1e-3) / 1e6,
             "wss_error_vs_f64_at_%d" % cmp_steps: cases.errors(w64, w16, wall), "mass_after": mass}
 

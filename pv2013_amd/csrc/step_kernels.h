@@ -786,7 +786,9 @@ __device__ __forceinline__ void segment_node(const StepArgs& a, int64_t s, int l
 // indexed by its lane within the piece; the two record masks join into one over
 // the wavefront (piece B's records follow piece A's in node order).  The rows
 // stay out of scalar and vector registers, so the pair path does not raise the
-// register count of the single-chunk path it shares a kernel with.
+// register count of the single-chunk path it shares a kernel with.  Pairs serve
+// the two-lattice pull only: its stores go to the wavefront's own nodes and
+// need no per-piece base.
 template <int Q, class P, int MODE, int SCH, bool LAT = false, bool TWO = false>
 __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lane, int32_t* srow = nullptr) {
     using L = Lat<Q>;
@@ -794,6 +796,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     using T = typename P::calc_t;
     constexpr int CW = lbm_chunk_w(Q);
     static_assert(2 * CW > 64 && 2 * CW <= 128, "two chunk rows take two LDS words per lane at most");
+    static_assert(!TWO || SCH == SCH_TWO, "chunk pairs serve the two-lattice pull, whose stores need no bases");
     const int32_t* h = a.chunk + c * CW;  // wave-uniform; piece B's row follows at h + CW
     if constexpr (TWO) {
         // every lane is still active here
@@ -837,12 +840,6 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     auto ivl = [&](int d) -> uint32_t {
         if constexpr (TWO) return ((uint32_t)lr[6 + Q + (d >> 1)] >> (16 * (d & 1))) & 0xffffu;
         else return (uint32_t)hl[d];
-    };
-    // the same base at the stores, read again rather than kept live across the
-    // collision (volatile: not merged with the read at the loads)
-    auto base_late = [&](int d) -> int32_t {
-        if constexpr (TWO) return ((const volatile int32_t*)lr)[6 + d];
-        else return hb[d];
     };
     T f[Q];
     if (!TWO && (mlo | mhi) == 0u) {
@@ -904,7 +901,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
         apply_bc<Q, P>(a, type, r, f);
         emit<Q, P, MODE, SCH, true, false, true>(a, n, f, [&](int d) {
             if (d == L::rest) return (int32_t)n;
-            return (pm >> L::opp(d)) & 1u ? base_late(L::opp(d)) + li : -1;
+            return (pm >> L::opp(d)) & 1u ? base(L::opp(d)) + li : -1;
         });
         return;
     }
@@ -928,7 +925,7 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
     // AA gather: the node at x + c_d is the source of direction opp d
     emit<Q, P, MODE, SCH, false>(a, n, f, [&](int d) {
         if (d == L::rest) return (int32_t)n;
-        return (pm >> L::opp(d)) & 1u ? base_late(L::opp(d)) + li : -1;
+        return (pm >> L::opp(d)) & 1u ? base(L::opp(d)) + li : -1;
     });
 }
 

@@ -725,6 +725,10 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             rm &= len == 64 ? ~0ull : ((1ull << len) - 1ull);
             h[4] = (int32_t)(uint32_t)rm;
             h[5] = (int32_t)(uint32_t)(rm >> 32);
+            // does any record lane of the chunk keep a stale population?
+            for (int l = 0; l < len; ++l)
+                if (((rm >> l) & 1ull) && bstale_mask[(size_t)h[2] + __builtin_popcountll(rm & ((1ull << l) - 1ull))])
+                    h[3] = 1;
             bool ok = len >= 1 && len <= 64 && (!word_chunks || sh + len <= 64);
             for (int e = 0; ok && e < q; ++e) {
                 if (e == rest) continue;

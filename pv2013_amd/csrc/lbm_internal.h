@@ -109,7 +109,8 @@ constexpr int LBM_SEG_W = 32;
 
 // Row chunk table entry (fused layout): every node belongs to one chunk, up to
 // 64 consecutive node ids along one row.
-//   [0] start node, [1] length, [2] records before start, [3] unused,
+//   [0] start node, [1] length, [2] records before start, [3] 1 if a record
+//   lane of the chunk keeps a stale population (an unlinked, unbounced link), else 0,
 //   [4], [5] record mask (bit l: node start + l has a boundary record),
 //   [6 + d] base_d,
 //   [6 + Q + d / 2] bits 16 (d & 1) .. 16 (d & 1) + 15: lo_d | hi_d << 8:

@@ -871,9 +871,11 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
                 pm |= (uint32_t)((uint32_t)li - lo < hi - lo) << d;
             }
         constexpr uint32_t nonrest = (uint32_t)((1ull << Q) - 1ull) & ~(1u << L::rest);
-        const uint32_t stn = isrec ? (~pm & ~bb & nonrest) : 0u;  // links that take the stale value
-        const bool anyst = __builtin_amdgcn_ballot_w64(stn != 0u) != 0ull;  // wave-uniform
+        // Whether any lane keeps a stale population comes from the chunk row,
+        // so the loads of the common chunks do not wait for the record words.
+        const bool anyst = (h[3] | (TWO ? h[CW + 3] : 0)) != 0;  // wave-uniform
         auto gather = [&](auto with_stale) {
+            const uint32_t stn = isrec ? (~pm & ~bb & nonrest) : 0u;  // links that take the stale value
 #pragma unroll
             for (int d = 0; d < Q; ++d) {
                 uint32_t e;
@@ -905,22 +907,30 @@ __device__ __forceinline__ void chunk_node(const StepArgs& a, int64_t c, int lan
         });
         return;
     }
+    // as above: a chunk whose lanes keep no stale population (the chunk row
+    // says so) picks between neighbour and own slot without the record words
+    const bool anyst = SCH == SCH_AA_LOCAL || (h[3] | (TWO ? h[CW + 3] : 0)) != 0;  // wave-uniform
+    auto gather = [&](auto with_stale) {
 #pragma unroll
-    for (int d = 0; d < Q; ++d) {
-        const S* st = stale + (int64_t)d * a.nrec + r;
-        const S* p;
-        if (SCH == SCH_AA_LOCAL || d == L::rest) {
-            p = ((lmiss & ~bb) >> d & 1u) ? st : gin + (int64_t)d * NS + n;
-        } else {
-            const uint32_t w = ivl(d);
-            const bool pres = (uint32_t)li >= (w & 0xffu) && (uint32_t)li < ((w >> 8) & 0xffu);
-            pm |= (uint32_t)pres << d;
-            const S* nb = gin + (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * NS + ((int64_t)base(d) + li);
-            const S* own = gin + (int64_t)(SCH == SCH_TWO ? L::opp(d) : d) * NS + n;
-            p = pres ? nb : ((bb >> d & 1u) ? own : st);
+        for (int d = 0; d < Q; ++d) {
+            const S* st = stale + (int64_t)d * a.nrec + r;
+            const S* p;
+            if (SCH == SCH_AA_LOCAL || d == L::rest) {
+                p = ((lmiss & ~bb) >> d & 1u) ? st : gin + (int64_t)d * NS + n;
+            } else {
+                const uint32_t w = ivl(d);
+                const bool pres = (uint32_t)li >= (w & 0xffu) && (uint32_t)li < ((w >> 8) & 0xffu);
+                pm |= (uint32_t)pres << d;
+                const S* nb = gin + (int64_t)(SCH == SCH_AA_GATHER ? L::opp(d) : d) * NS + ((int64_t)base(d) + li);
+                const S* own = gin + (int64_t)(SCH == SCH_TWO ? L::opp(d) : d) * NS + n;
+                if constexpr (decltype(with_stale)::value) p = pres ? nb : ((bb >> d & 1u) ? own : st);
+                else p = pres ? nb : own;
+            }
+            f[d] = (T)*p;
         }
-        f[d] = (T)*p;
-    }
+    };
+    if (anyst) gather(std::true_type{});
+    else gather(std::false_type{});
     apply_bc<Q, P>(a, type, r, f);
     // AA gather: the node at x + c_d is the source of direction opp d
     emit<Q, P, MODE, SCH, false>(a, n, f, [&](int d) {

@@ -79,7 +79,10 @@
  *   orc_sim_step         Simulation.cpp:903-932: Collision, Streaming, inlet,
  *                        wall, outlets, in that order, over the node lists.  Each
  *                        part is pinned as above and every node belongs to one
- *                        list, so the list order cannot change a value; the whole
+ *                        list, so the list order cannot change a value: the three
+ *                        boundary passes run separately (orc_sim_step_passes), in
+ *                        the reference's order and in the five others, equal the
+ *                        per-node loop bit for bit (tests/test_oracle_pins.py); the whole
  *                        is pinned by mass conservation, the resting box, the
  *                        shear wave and config 1's Poiseuille profile with the
  *                        Hagen-Poiseuille pressure gradient (clean bounce-back).
@@ -742,6 +745,67 @@ static void step_once(orc_sim* s) {
 
 void orc_sim_step(orc_sim* s, int64_t nsteps) {
     for (int64_t i = 0; i < nsteps; ++i) step_once(s);
+}
+
+/* The boundary part of step_once as the reference runs it: three passes over
+ * separate node lists, ApplyZouHeBc over the inlet list, ApplyWallBc over the
+ * wall list, ApplyOutletBc over the outlet lists (Simulation.cpp:908-918), in
+ * the order order[0..2] (0 inlet, 1 wall, 2 outlet; the reference's is 0 1 2).
+ * Only for quirks = 1, where the lists are disjoint.  Tests compare every order
+ * with step_once's single per-node loop. */
+int orc_sim_step_passes(orc_sim* s, int64_t nsteps, const int32_t order[3]) {
+    if (!s->quirks) return -1;
+    int seen = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (order[k] < 0 || order[k] > 2) return -1;
+        seen |= 1 << order[k];
+    }
+    if (seen != 7) return -1;
+    const int q = s->q;
+    const int32_t* opp = opps(q);
+    const int64_t n = s->n;
+    const int rd = rest_dir(q);
+    const int d0 = q == 19 ? 1 : 0;
+    for (int64_t i = 0; i < nsteps; ++i) {
+        double rho, u[3];
+        for (int64_t m = 0; m < n; ++m) orc_collide_node(q, s->omega, s->f + m * q, s->fnew + m * q, &rho, u);
+        for (int64_t m = 0; m < n; ++m)
+            for (int d = 0; d < q; ++d) {
+                int64_t t = s->nb[m * q + d];
+                if (t >= 0 && s->f[m * q + d] != 0.0) s->f[t * q + d] = s->fnew[m * q + d];
+            }
+        double wv[MAXWAVE];
+        for (int w = 0; w < MAXWAVE; ++w) wv[w] = 1.0;
+        for (int b = 0; b < s->nbc; ++b)
+            if (s->bcs[b].waveform >= 0 && s->bcs[b].waveform < MAXWAVE)
+                wv[s->bcs[b].waveform] = wave_eval(s, s->bcs[b].waveform, s->iter);
+        for (int k = 0; k < 3; ++k) {
+            const int pass = order[k];
+            for (int64_t m = 0; m < n; ++m) {
+                const int t = s->type[m];
+                if (pass == 1) {
+                    if (t != 2) continue;
+                    for (int d = d0; d < q; ++d) {
+                        if (d == rd) continue;
+                        if (s->nb[m * q + d] < 0) s->f[m * q + opp[d]] = s->fnew[m * q + d];
+                    }
+                    continue;
+                }
+                if (t < 3 || t - 3 >= s->nbc) continue;
+                const orc_bc* bc = &s->bcs[t - 3];
+                if ((pass == 0) != (bc->kind == ORC_BC_VELOCITY)) continue;
+                double uv[3] = {0, 0, 0};
+                if (bc->kind == ORC_BC_VELOCITY) {
+                    double w = bc->waveform >= 0 ? wv[bc->waveform] : 1.0;
+                    double fac = s->prof[m] * w;
+                    for (int a = 0; a < 3; ++a) uv[a] = bc->u[a] * fac;
+                }
+                orc_zouhe_node(q, bc, uv, s->f + m * q);
+            }
+        }
+        s->iter++;
+    }
+    return 0;
 }
 
 /* The same step with OpenMP over the nodes of each pass, for the timed CPU

@@ -89,6 +89,9 @@ void orc_sim_destroy(orc_sim* s);
 int orc_sim_set_waveform(orc_sim* s, int slot, int kind, double period, int nk,
                          const double* a, const double* phi, double dt);
 void orc_sim_step(orc_sim* s, int64_t nsteps);
+/* The same step with the boundary lists as three separate passes in a given
+ * order (0 inlet, 1 wall, 2 outlet); quirks = 1 only.  0, or -1 on bad input. */
+int orc_sim_step_passes(orc_sim* s, int64_t nsteps, const int32_t order[3]);
 /* The same steps with OpenMP threads (timed CPU baseline); identical results
  * whenever no population is exactly 0.0.  Returns 0, or -12 without memory. */
 int orc_sim_step_threads(orc_sim* s, int64_t nsteps, int nthreads);

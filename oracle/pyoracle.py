@@ -64,6 +64,7 @@ def lib():
         L.orc_sim_set_waveform.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
                                            P(np.float64), P(np.float64), C.c_double]
         L.orc_sim_step.argtypes = [C.c_void_p, C.c_int64]
+        L.orc_sim_step_passes.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
         L.orc_sim_step_threads.argtypes = [C.c_void_p, C.c_int64, C.c_int]
         L.orc_sim_step_threads.restype = C.c_int
         L.orc_threads_available.restype = C.c_int
@@ -208,6 +209,12 @@ class Sim:
 
     def step(self, n=1):
         lib().orc_sim_step(self.h, n)
+
+    def step_passes(self, n, order):
+        """n steps with the boundary lists as separate passes in `order`
+        (0 inlet, 1 wall, 2 outlet), as the reference runs them."""
+        o = (C.c_int32 * 3)(*order)
+        assert lib().orc_sim_step_passes(self.h, n, o) == 0
 
     def step_threads(self, n=1, threads=0):
         """The same steps on OpenMP threads (0: the OpenMP default)."""

@@ -756,7 +756,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
             for (int64_t c = 0; c < D->nchunk;) {
                 const int32_t* h = &chunks[c * CW];
                 const bool two = c + 1 < D->nchunk && (h[CW] >> 6) == (h[0] >> 6);
-                if (two && (h[CW] != h[0] + h[1] || h[1] + h[CW + 1] > 64 || c >= LBM_WAVE_PAIR)) {
+                if (c >= LBM_WAVE_PAIR || (two && (h[CW] != h[0] + h[1] || h[1] + h[CW + 1] > 64))) {
                     delete D;
                     return lbm_ctx_fail(ctx, LBM_EIO, "internal error: bad chunk pair");
                 }

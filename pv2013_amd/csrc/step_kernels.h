@@ -1050,9 +1050,10 @@ void launch_layout(int layout, const StepArgs& b, hipStream_t s) {
             c.wreg = nullptr;
             c.nblocks = (int)((b.n + b.block - 1) / b.block);
             hipLaunchKernelGGL((k_step<Q, P, MODE, SCH>), dim3(lbm_grid_blocks(b.n, b.block)), dim3(b.block), 0, s, c);
-        } else if constexpr (sizeof(typename P::store_t) == 2) {
-            // 16-bit storage: chunks with records through one lattice-wide
-            // descriptor while the lattice is below 4 GiB
+        } else if constexpr (sizeof(typename P::store_t) == 2 ||
+                             (sizeof(typename P::store_t) == 4 && SCH == SCH_TWO)) {
+            // 16-bit storage, and the FP32 pull: chunks with records through
+            // one lattice-wide descriptor while the lattice is below 4 GiB
             if (b.wide)
                 hipLaunchKernelGGL((k_step_rs<Q, P, MODE, SCH, PART_FUSED, false>),
                                    dim3(lbm_grid_blocks(fused_waves(b, SCH), b.block / 64)), dim3(b.block), 0, s, b);

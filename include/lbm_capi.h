@@ -175,8 +175,9 @@ typedef struct {
     int32_t storage;  /* LBM_STORE_* */
     int32_t scheme;   /* LBM_SCHEME_* */
     int32_t quirks;   /* 1: the reference's step semantics; 0: clean bounce-back */
-    int32_t block;    /* step-kernel threads per block: 64, 128, 256 or 512; 0 = 64 for
-                         FP16 storage, 512 otherwise (the fastest on the benchmark case) */
+    int32_t block;    /* step-kernel threads per block: 64, 128, 256 or 512; 0 = the
+                         fastest on the benchmark case: 64 for FP16 storage; AA 512;
+                         two-lattice pull 128 (FP32) or 256 (FP64) */
     double tau;
     const uint8_t* flags;    /* host, dims product */
     const lbm_bc_desc* bcs;  /* bcs[b] drives flag 3+b */
@@ -294,7 +295,7 @@ typedef struct {
 
 /* Times each candidate layout of `desc` (streaming scheme two-lattice / AA,
  * node-order fast axis x / y / z, kernel: row chunks in one launch / records
- * and segments in two / generic, at the default block size for the storage;
+ * and segments in two / generic, at the default block size for the storage and scheme;
  * then the other block sizes of 64 / 128 / 256 / 512 for the winner) for `steps` steps
  * after a warm-up and returns the fastest.  When cache_path is not NULL the
  * result is looked up there first (key: lattice, storage, dims, flag hash) and

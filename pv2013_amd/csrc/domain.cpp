@@ -239,8 +239,13 @@ struct lbm_domain {
 static size_t store_bytes(int storage) { return storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2); }
 
 // Step-kernel threads per block when the description leaves it 0: the fastest
-// on the benchmark case (FP16 storage: 64; FP32 and FP64: 512).
-static int default_block(int storage) { return storage == LBM_STORE_F16S ? 64 : 512; }
+// on the benchmark case per storage and scheme (FP16: 64; AA: 512 for FP32 and
+// FP64; two-lattice pull: 128 for FP32, 256 for FP64).
+static int default_block(int storage, int scheme) {
+    if (storage == LBM_STORE_F16S) return 64;
+    if (scheme == LBM_SCHEME_TWO_LATTICE) return storage == LBM_STORE_F32 ? 128 : 256;
+    return 512;
+}
 
 static int choose_fast_axis(const uint8_t* flags, const int* dims) {
     // fewest runs per fluid site = longest runs
@@ -441,7 +446,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     D->scheme = d.scheme;
     D->quirks = d.quirks;
     D->tau = d.tau;
-    D->block = d.block > 0 ? d.block : default_block(d.storage);
+    D->block = d.block > 0 ? d.block : default_block(d.storage, d.scheme);
     if (D->block != 64 && D->block != 128 && D->block != 256 && D->block != 512) {
         delete D;
         return lbm_ctx_fail(ctx, LBM_EINVAL, "block must be 64, 128, 256 or 512 threads");
@@ -1526,11 +1531,11 @@ extern "C" int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps
     for (int sch = 0; sch < 2; ++sch)
         for (int ax = 0; ax < 3; ++ax)
             for (int kern = 0; kern < 3; ++kern)
-                if (int rc = trial(sch, ax, keep | kern_opt[kern], default_block(desc->storage))) return rc;
+                if (int rc = trial(sch, ax, keep | kern_opt[kern], default_block(desc->storage, sch))) return rc;
     // stage 2: the other block sizes for the winner
     const int bs = out->scheme, ba = out->fast_axis, bo = out->options;
     for (int blk : {64, 128, 256, 512})
-        if (blk != default_block(desc->storage))
+        if (blk != default_block(desc->storage, bs))
             if (int rc = trial(bs, ba, bo, blk)) return rc;
     if (cache_path) {
         if (FILE* fp = std::fopen(cache_path, "a")) {

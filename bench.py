@@ -131,8 +131,9 @@ def main():
     ap.add_argument("--scheme", default="aa", choices=["two", "aa"],
                     help="streaming: AA (one lattice, default) or two-lattice pull; identical results")
     ap.add_argument("--block", type=int, default=0, choices=[0, 64, 128, 256, 512],
-                    help="threads per block of the step kernels (0: the library default, 64 for FP16 "
-                         "storage and 512 otherwise)")
+                    help="threads per block of the step kernels (0: the library default: 64 for FP16 "
+                         "storage; two-lattice 128 for FP32, 256 for FP64; AA 512 while the lattice fits "
+                         "the Infinity Cache, else 256)")
     ap.add_argument("--layout", default="fused", choices=["fused", "split", "generic"],
                     help="step kernel: row chunks in one launch (default), records + row segments "
                          "in two, or the generic per-node kernel")

@@ -176,8 +176,9 @@ typedef struct {
     int32_t scheme;   /* LBM_SCHEME_* */
     int32_t quirks;   /* 1: the reference's step semantics; 0: clean bounce-back */
     int32_t block;    /* step-kernel threads per block: 64, 128, 256 or 512; 0 = the
-                         fastest on the benchmark case: 64 for FP16 storage; AA 512;
-                         two-lattice pull 128 (FP32) or 256 (FP64) */
+                         measured default: 64 for FP16 storage; two-lattice pull 128
+                         (FP32) or 256 (FP64); AA 512 while the lattice fits the 256 MiB
+                         Infinity Cache, else 256 */
     double tau;
     const uint8_t* flags;    /* host, dims product */
     const lbm_bc_desc* bcs;  /* bcs[b] drives flag 3+b */

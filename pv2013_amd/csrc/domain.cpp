@@ -238,13 +238,15 @@ struct lbm_domain {
 
 static size_t store_bytes(int storage) { return storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2); }
 
-// Step-kernel threads per block when the description leaves it 0: the fastest
-// on the benchmark case per storage and scheme (FP16: 64; AA: 512 for FP32 and
-// FP64; two-lattice pull: 128 for FP32, 256 for FP64).
-static int default_block(int storage, int scheme) {
+// Step-kernel threads per block when the description leaves it 0, from
+// measurements on the benchmark configurations: FP16 storage 64; the
+// two-lattice pull 128 (FP32) or 256 (FP64); AA 512 while its lattice fits the
+// 256 MiB Infinity Cache (the 209 MB FP32 benchmark lattice), 256 once it
+// streams from HBM (the 2.6 GB FP32 ensemble, the 417 MB FP64 lattice).
+static int default_block(int storage, int scheme, uint64_t lattice_bytes) {
     if (storage == LBM_STORE_F16S) return 64;
     if (scheme == LBM_SCHEME_TWO_LATTICE) return storage == LBM_STORE_F32 ? 128 : 256;
-    return 512;
+    return lattice_bytes <= ((uint64_t)256 << 20) ? 512 : 256;
 }
 
 static int choose_fast_axis(const uint8_t* flags, const int* dims) {
@@ -446,8 +448,8 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     D->scheme = d.scheme;
     D->quirks = d.quirks;
     D->tau = d.tau;
-    D->block = d.block > 0 ? d.block : default_block(d.storage, d.scheme);
-    if (D->block != 64 && D->block != 128 && D->block != 256 && D->block != 512) {
+    D->block = d.block;  // 0: the default, chosen below once the lattice size is known
+    if (D->block != 0 && D->block != 64 && D->block != 128 && D->block != 256 && D->block != 512) {
         delete D;
         return lbm_ctx_fail(ctx, LBM_EINVAL, "block must be 64, 128, 256 or 512 threads");
     }
@@ -820,6 +822,7 @@ extern "C" int lbm_domain_create(lbm_ctx* ctx, const lbm_domain_desc* desc, lbm_
     DCHECK(D->alloc(&D->wvals, (size_t)LBM_MAXWAVE * LBM_WCHUNK));
     uint8_t *g0 = nullptr, *g1 = nullptr;
     D->ns = (D->n + 63) / 64 * 64;
+    if (D->block == 0) D->block = default_block(d.storage, d.scheme, (uint64_t)q * (uint64_t)D->ns * sb);
     DCHECK(D->alloc(&g0, (size_t)q * D->ns * sb));
     if (d.scheme == LBM_SCHEME_TWO_LATTICE) DCHECK(D->alloc(&g1, (size_t)q * D->ns * sb));
     D->g[0] = g0;
@@ -1511,6 +1514,7 @@ extern "C" int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps
         rc = lbm_step(D, 64);
         if (!rc) rc = lbm_step_timed(D, steps, &ms);
         const int64_t n = D->n;
+        const int used_block = D->block;  // blk, or the domain's default when blk is 0
         lbm_domain_destroy(D);
         if (rc) return rc;
         const double mlups = ms > 0 ? (double)n * steps / (ms * 1e-3) / 1e6 : 0.0;
@@ -1520,22 +1524,22 @@ extern "C" int lbm_autotune(lbm_ctx* ctx, const lbm_domain_desc* desc, int steps
             out->scheme = sch;
             out->fast_axis = ax;
             out->options = opts;
-            out->block = blk;
+            out->block = used_block;
             out->mlups = mlups;
         }
         return LBM_OK;
     };
-    // stage 1: scheme x node order x kernel layout at the storage's default block size
+    // stage 1: scheme x node order x kernel layout at the default block size
     static const int kern_opt[3] = {0, LBM_OPT_SPLIT_LAUNCH, LBM_OPT_GENERIC_KERNEL};
     const int keep = desc->options & ~(LBM_OPT_NO_ROW_FASTPATH | LBM_OPT_GENERIC_KERNEL | LBM_OPT_SPLIT_LAUNCH);
     for (int sch = 0; sch < 2; ++sch)
         for (int ax = 0; ax < 3; ++ax)
             for (int kern = 0; kern < 3; ++kern)
-                if (int rc = trial(sch, ax, keep | kern_opt[kern], default_block(desc->storage, sch))) return rc;
+                if (int rc = trial(sch, ax, keep | kern_opt[kern], 0)) return rc;
     // stage 2: the other block sizes for the winner
-    const int bs = out->scheme, ba = out->fast_axis, bo = out->options;
+    const int bs = out->scheme, ba = out->fast_axis, bo = out->options, bb = out->block;
     for (int blk : {64, 128, 256, 512})
-        if (blk != default_block(desc->storage, bs))
+        if (blk != bb)
             if (int rc = trial(bs, ba, bo, blk)) return rc;
     if (cache_path) {
         if (FILE* fp = std::fopen(cache_path, "a")) {

@@ -385,6 +385,10 @@ static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* 
     a.wk = wk;
     a.gin = gin;
     a.gout = gout;
+    // AA: the gather step (sch 1) walks the lattice downward, so each launch
+    // starts on the lines the previous one touched last (the two-lattice pull
+    // keeps one direction: alternating it measured slower in FP32)
+    a.rev = D->scheme == LBM_SCHEME_AA && sch == 1 && !(D->options & LBM_OPT_FORWARD_ONLY);
     a.fpre = D->fpre;
     a.rho = D->rho;
     a.u = D->u;

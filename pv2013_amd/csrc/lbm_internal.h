@@ -65,6 +65,7 @@ struct StepArgs {
     const FaceTab* faces;
     const double* wvals;    // [slot][LBM_WCHUNK] waveform values of the current chunk
     int32_t wk;             // this launch's row in the chunk
+    int32_t rev;            // 1: logical blocks in descending order within each XCD
     const void* gin;        // [d][ns] post-collision populations
     void* gout;
     double omega;

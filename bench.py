@@ -178,10 +178,11 @@ def main():
     # practical ceiling of this layout), in place and into a second lattice
     probes = {k: ctx.bandwidth_lattice(19, sbytes, n, pattern=v, reps=10)
               for k, v in (("in_place", P.capi.PROBE_IN_PLACE), ("in_place_shifted", P.capi.PROBE_IN_PLACE_SHIFTED),
-                           ("two", P.capi.PROBE_TWO), ("two_shifted", P.capi.PROBE_TWO_SHIFTED))}
-    # the AA step alternates the gather (the shifted in-place pattern) and the
-    # local step (in place, aligned): its ceiling is the mean of the two times
-    probe_aa = 2.0 / (1.0 / probes["in_place"] + 1.0 / probes["in_place_shifted"])
+                           ("two", P.capi.PROBE_TWO), ("two_shifted", P.capi.PROBE_TWO_SHIFTED),
+                           ("aa", P.capi.PROBE_AA))}
+    # the AA step alternates the gather (the shifted in-place pattern, walking
+    # downward) and the local step (in place, aligned, walking upward)
+    probe_aa = probes["aa"]
     probe_two = probes["two_shifted"]
     extra["measured_gbs"] = {"in_place_lattice_size": ctx.bandwidth(n * 19 * sbytes, in_place=True),
                              "copy_1GiB": ctx.bandwidth(1 << 30, in_place=False, reps=5),
@@ -255,8 +256,9 @@ def main():
                      "practical_ceiling": {
                          "probe": "lbm_ctx_bandwidth_lattice: the step's 19 arrays of this node count and storage "
                                   "read then written with no arithmetic, "
-                                  + ("in place, half the passes with every run one node off its line (the AA "
-                                     "gather's pattern), half aligned (the local step's)" if scheme == P.SCHEME_AA
+                                  + ("in place, alternating a pass with every run one node off its line walking "
+                                     "the lattice downward (the AA gather's pattern and order) and an aligned pass "
+                                     "walking it upward (the local step's)" if scheme == P.SCHEME_AA
                                      else "into a second lattice, reads one node off their lines (the pull's "
                                           "pattern)") + "; aligned_*: every access aligned",
                          "gbs": probe_aa if scheme == P.SCHEME_AA else probe_two,

@@ -73,13 +73,19 @@ int lbm_ctx_bandwidth(lbm_ctx* ctx, int64_t bytes, int in_place, int reps, doubl
  *                              so every 64-lane read run starts off its cache
  *                              line as a neighbour run does (the pull);
  *   LBM_PROBE_IN_PLACE_SHIFTED (3)  in place, reads and writes one node along
- *                              (the AA gather).
- * Repeated `reps` times after one warm-up pass; *gbs = bytes read + written
- * per second, GB/s, from HIP events.  No reference counterpart. */
+ *                              (the AA gather);
+ *   LBM_PROBE_AA (4)           the AA step's pair of launches: a shifted pass
+ *                              in place walking the lattice downward (the
+ *                              gather), then an aligned one walking it upward
+ *                              (the local step).
+ * Patterns 0-3 walk upward.  Repeated `reps` times (pattern 4: pairs) after one
+ * warm-up; *gbs = bytes read + written per second, GB/s, from HIP events.  No
+ * reference counterpart. */
 #define LBM_PROBE_TWO 0
 #define LBM_PROBE_IN_PLACE 1
 #define LBM_PROBE_TWO_SHIFTED 2
 #define LBM_PROBE_IN_PLACE_SHIFTED 3
+#define LBM_PROBE_AA 4
 int lbm_ctx_bandwidth_lattice(lbm_ctx* ctx, int q, int elem_bytes, int64_t nodes, int pattern, int reps,
                               double* gbs);
 int lbm_version(void);

@@ -21,7 +21,7 @@ OPT_PERIODIC_X, OPT_PERIODIC_Y, OPT_PERIODIC_Z = 4, 8, 16
 OPT_GENERIC_KERNEL = 32  # one thread per node through the site map
 OPT_SPLIT_LAUNCH = 64  # records and row segments in two launches
 OPT_WIDE_RECORDS = 128  # FP16 chunks with records through 64-bit addresses (automatic >= 4 GiB)
-PROBE_TWO, PROBE_IN_PLACE, PROBE_TWO_SHIFTED, PROBE_IN_PLACE_SHIFTED = 0, 1, 2, 3  # lbm_ctx_bandwidth_lattice
+PROBE_TWO, PROBE_IN_PLACE, PROBE_TWO_SHIFTED, PROBE_IN_PLACE_SHIFTED, PROBE_AA = 0, 1, 2, 3, 4  # lbm_ctx_bandwidth_lattice
 OPT_ONE_CHUNK_PER_WAVE = 256  # two-lattice row chunks one per wavefront instead of two-chunk windows
 OPT_FORWARD_ONLY = 512  # every step launch walks the lattice upward (default: the AA gather walks it downward)
 BC_VELOCITY, BC_PRESSURE = 0, 1
@@ -247,7 +247,8 @@ class Context:
     def bandwidth_lattice(self, q, elem_bytes, nodes, pattern=1, reps=20):
         """Measured GB/s of a step's traffic with no arithmetic: q arrays of
         nodes elements, read then written (lbm_ctx_bandwidth_lattice; pattern
-        PROBE_TWO, PROBE_IN_PLACE, PROBE_TWO_SHIFTED or PROBE_IN_PLACE_SHIFTED)."""
+        PROBE_TWO, PROBE_IN_PLACE, PROBE_TWO_SHIFTED, PROBE_IN_PLACE_SHIFTED or
+        PROBE_AA)."""
         g = C.c_double()
         self.check(lib().lbm_ctx_bandwidth_lattice(self.h, q, elem_bytes, int(nodes), int(pattern), reps,
                                                    C.byref(g)), "lbm_ctx_bandwidth_lattice")

@@ -79,10 +79,12 @@ hipError_t launch_stream_probe(float4* a, float4* b, int64_t n, hipStream_t s) {
 // pull's pattern).  SHIFT = 1 moves the reads (and, in place, the writes) one
 // node along, so every 64-lane run starts off its cache line as a neighbour
 // run does (the pull's reads; the AA gather's reads and writes).  Integer adds
-// keep every store live.
+// keep every store live.  Blocks map to nodes as in the step kernels (a
+// contiguous range per XCD; 8-byte elements: one front), walked downward when
+// rev is set.
 template <int Q, class E, int SHIFT>
-__global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, int64_t ns) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, int64_t ns, int rev) {
+    const int64_t i = logical_block(rev, sizeof(E) == 8) * 256 + threadIdx.x;
     if (i >= n) return;
     const int64_t j = i + SHIFT;  // < ns: the stride leaves room for the shift
     E v[Q];
@@ -95,24 +97,26 @@ __global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, in
 }
 
 template <class E, int SHIFT>
-static hipError_t launch_lattice_probe_t(int q, void* a, void* b, int64_t n, int64_t ns, hipStream_t s) {
-    const dim3 grid((unsigned)((n + 255) / 256));
-    if (q == 19) hipLaunchKernelGGL((k_lattice_probe<19, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns);
-    else hipLaunchKernelGGL((k_lattice_probe<27, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns);
+static hipError_t launch_lattice_probe_t(int q, int rev, void* a, void* b, int64_t n, int64_t ns, hipStream_t s) {
+    const dim3 grid((unsigned)lbm_grid_blocks(n, 256));
+    if (q == 19)
+        hipLaunchKernelGGL((k_lattice_probe<19, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns, rev);
+    else
+        hipLaunchKernelGGL((k_lattice_probe<27, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns, rev);
     return hipGetLastError();
 }
 
-hipError_t launch_lattice_probe(int q, int elem_bytes, int shift, void* a, void* b, int64_t n, int64_t ns,
+hipError_t launch_lattice_probe(int q, int elem_bytes, int shift, int rev, void* a, void* b, int64_t n, int64_t ns,
                                 hipStream_t s) {
     if (!a || n <= 0 || ns < n + shift || (shift != 0 && shift != 1) || (q != 19 && q != 27) ||
-        (n + 255) / 256 > INT32_MAX)
+        (n + 255) / 256 > INT32_MAX - 8)
         return hipErrorInvalidValue;
-    if (elem_bytes == 2) return shift ? launch_lattice_probe_t<uint16_t, 1>(q, a, b, n, ns, s)
-                                      : launch_lattice_probe_t<uint16_t, 0>(q, a, b, n, ns, s);
-    if (elem_bytes == 4) return shift ? launch_lattice_probe_t<uint32_t, 1>(q, a, b, n, ns, s)
-                                      : launch_lattice_probe_t<uint32_t, 0>(q, a, b, n, ns, s);
-    if (elem_bytes == 8) return shift ? launch_lattice_probe_t<uint64_t, 1>(q, a, b, n, ns, s)
-                                      : launch_lattice_probe_t<uint64_t, 0>(q, a, b, n, ns, s);
+    if (elem_bytes == 2) return shift ? launch_lattice_probe_t<uint16_t, 1>(q, rev, a, b, n, ns, s)
+                                      : launch_lattice_probe_t<uint16_t, 0>(q, rev, a, b, n, ns, s);
+    if (elem_bytes == 4) return shift ? launch_lattice_probe_t<uint32_t, 1>(q, rev, a, b, n, ns, s)
+                                      : launch_lattice_probe_t<uint32_t, 0>(q, rev, a, b, n, ns, s);
+    if (elem_bytes == 8) return shift ? launch_lattice_probe_t<uint64_t, 1>(q, rev, a, b, n, ns, s)
+                                      : launch_lattice_probe_t<uint64_t, 0>(q, rev, a, b, n, ns, s);
     return hipErrorInvalidValue;
 }
 

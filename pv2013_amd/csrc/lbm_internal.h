@@ -65,7 +65,8 @@ struct StepArgs {
     const FaceTab* faces;
     const double* wvals;    // [slot][LBM_WCHUNK] waveform values of the current chunk
     int32_t wk;             // this launch's row in the chunk
-    int32_t rev;            // 1: logical blocks in descending order within each XCD
+    int32_t rev;            // 1: logical blocks in descending order (within each XCD's range)
+    int32_t front;          // 1: logical blocks in dispatch order (no per-XCD ranges)
     const void* gin;        // [d][ns] post-collision populations
     void* gout;
     double omega;
@@ -134,8 +135,10 @@ hipError_t launch_stream_probe(float4* a, float4* b, int64_t n, hipStream_t s);
 hipError_t launch_stream_fill(float4* a, int64_t n, hipStream_t s);
 // lattice-shaped probe: q arrays of elem_bytes elements, stride ns; b == nullptr:
 // in place (reversed array order), else into lattice b; shift 1: reads (in place:
-// reads and writes) one node along, off their cache lines (needs ns >= n + 1)
-hipError_t launch_lattice_probe(int q, int elem_bytes, int shift, void* a, void* b, int64_t n, int64_t ns,
+// reads and writes) one node along, off their cache lines (needs ns >= n + 1);
+// rev 1: the blocks walk the lattice downward; 8-byte elements map blocks as the
+// FP64 step kernels do (one front, no per-XCD ranges)
+hipError_t launch_lattice_probe(int q, int elem_bytes, int shift, int rev, void* a, void* b, int64_t n, int64_t ns,
                                 hipStream_t s);
 hipError_t launch_wave_chunk(const WaveTab* waves, const int32_t* used, int nused, const int64_t* iter,
                              int64_t it0, int it_abs, double* wvals, hipStream_t s);

@@ -39,9 +39,12 @@ struct PrecF16S {
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of logical blocks so neighbouring rows share an L2.
-// rev: each XCD walks its range from the top, so a launch starts on the lines
-// the previous launch touched last.
-__device__ __forceinline__ int64_t logical_block(int rev = 0) {
+// front: logical block = dispatch order instead, one front through the
+// lattice (FP64, whose 64-lane accesses fill whole lines by themselves).
+// rev: the walk starts from the top, so a launch starts on the lines the
+// previous launch touched last.
+__device__ __forceinline__ int64_t logical_block(int rev = 0, int front = 0) {
+    if (front) return rev ? (int64_t)gridDim.x - 1 - blockIdx.x : (int64_t)blockIdx.x;
     const int per = gridDim.x >> 3;
     const int k = (int)(blockIdx.x >> 3);
     return (int64_t)(blockIdx.x & 7) * per + (rev ? per - 1 - k : k);
@@ -621,7 +624,7 @@ __global__ __launch_bounds__(512) void k_step(StepArgs a) {
     using L = Lat<Q>;
     using S = typename P::store_t;
     using T = typename P::calc_t;
-    const int64_t lb = logical_block(a.rev);
+    const int64_t lb = logical_block(a.rev, a.front);
     const int64_t n = lb * blockDim.x + threadIdx.x;
     if (lb >= a.nblocks || n >= a.n) return;
     // gin and gout are the same buffer in the AA scheme, where a thread
@@ -951,7 +954,7 @@ template <int Q, class P, int MODE, int SCH, int PART, bool LAT = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     Q == 27 && sizeof(typename P::store_t) == 2 ? 5 : 1))) void k_step_rs(StepArgs a) {
     if constexpr (PART == PART_FUSED) {
-        const int64_t lb = logical_block(a.rev);
+        const int64_t lb = logical_block(a.rev, a.front);
         if (lb >= a.nblocks) return;
         const int64_t j = __builtin_amdgcn_readfirstlane((int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6)));
         const int lane = (int)(threadIdx.x & 63);

@@ -98,12 +98,10 @@ def run_c4(ctx, quick):
 
 
 def aa_ceiling(ctx, q, eb, nodes):
-    """GB/s of the AA step's traffic with no arithmetic: the mean time of the
-    shifted in-place pass (the gather's runs start one node off their lines)
-    and the aligned one (the local step)."""
-    a = ctx.bandwidth_lattice(q, eb, nodes, pattern=P.PROBE_IN_PLACE, reps=5)
-    b = ctx.bandwidth_lattice(q, eb, nodes, pattern=P.PROBE_IN_PLACE_SHIFTED, reps=5)
-    return 2.0 / (1.0 / a + 1.0 / b)
+    """GB/s of the AA step's traffic with no arithmetic: pairs of in-place
+    passes, a shifted one walking downward (the gather's runs start one node
+    off their lines) and an aligned one walking upward (the local step)."""
+    return ctx.bandwidth_lattice(q, eb, nodes, pattern=P.PROBE_AA, reps=5)
 
 
 def run_c5(ctx, quick, steps=None):

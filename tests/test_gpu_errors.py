@@ -117,13 +117,14 @@ def test_bandwidth_probes(P, ctx):
     assert ctx.bandwidth(1 << 20, in_place=False, reps=2) > 0
     for q in (19, 27):
         for eb in (2, 4, 8):
-            for pattern in range(4):
+            for pattern in range(5):
                 assert ctx.bandwidth_lattice(q, eb, 10007, pattern=pattern, reps=2) > 0
     # a node count one below a multiple of 64: the shifted patterns' last read
     # is the stride's last element
     assert ctx.bandwidth_lattice(19, 4, 4095, pattern=3, reps=1) > 0
+    assert ctx.bandwidth_lattice(19, 4, 4095, pattern=4, reps=1) > 0
     expect(P, lambda: ctx.bandwidth_lattice(15, 4, 1000), -22, "bad argument")
     expect(P, lambda: ctx.bandwidth_lattice(19, 3, 1000), -22, "bad argument")
     expect(P, lambda: ctx.bandwidth_lattice(19, 4, 0), -22, "bad argument")
-    expect(P, lambda: ctx.bandwidth_lattice(19, 4, 1000, pattern=4), -22, "bad argument")
+    expect(P, lambda: ctx.bandwidth_lattice(19, 4, 1000, pattern=5), -22, "bad argument")
     assert ctx.bandwidth_lattice(19, 4, 4096, reps=1) > 0

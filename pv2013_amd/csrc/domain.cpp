@@ -396,10 +396,11 @@ static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* 
     // starts on the lines the previous one touched last (the two-lattice pull
     // keeps one direction: alternating it measured slower in FP32)
     a.rev = D->scheme == LBM_SCHEME_AA && sch == 1 && !(D->options & LBM_OPT_FORWARD_ONLY);
-    // FP64 walks one front through the lattice instead of a range per XCD:
-    // benchmark case AA 162.6 -> 149.0, two-lattice 172.9 -> 165.6 us per step;
-    // FP32 AA (65.95 against 71.51) and FP16 (38.9 against 55.6) keep the ranges
-    a.front = D->storage == LBM_STORE_F64;
+    // lattices larger than the Infinity Cache stream as one front of 16-block
+    // tiles (64-pipe ensemble FP32 1140 -> 1067 us per step), cache-resident
+    // ones keep one range per XCD (benchmark case FP32 AA 66.0 against 68.5)
+    a.tile = lbm_launch_tile((uint64_t)D->q * (uint64_t)D->ns * store_bytes(D->storage) *
+                             (D->scheme == LBM_SCHEME_TWO_LATTICE ? 2u : 1u));
     a.fpre = D->fpre;
     a.rho = D->rho;
     a.u = D->u;

@@ -79,12 +79,12 @@ hipError_t launch_stream_probe(float4* a, float4* b, int64_t n, hipStream_t s) {
 // pull's pattern).  SHIFT = 1 moves the reads (and, in place, the writes) one
 // node along, so every 64-lane run starts off its cache line as a neighbour
 // run does (the pull's reads; the AA gather's reads and writes).  Integer adds
-// keep every store live.  Blocks map to nodes as in the step kernels (a
-// contiguous range per XCD; 8-byte elements: one front), walked downward when
-// rev is set.
+// keep every store live.  Blocks map to nodes as in the step kernels (tile:
+// one range per XCD, or tiles in turn for lattices beyond the Infinity
+// Cache), walked downward when rev is set.
 template <int Q, class E, int SHIFT>
-__global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, int64_t ns, int rev) {
-    const int64_t i = logical_block(rev, sizeof(E) == 8) * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, int64_t ns, int rev, int tile) {
+    const int64_t i = logical_block(rev, tile) * 256 + threadIdx.x;
     if (i >= n) return;
     const int64_t j = i + SHIFT;  // < ns: the stride leaves room for the shift
     E v[Q];
@@ -99,10 +99,11 @@ __global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, in
 template <class E, int SHIFT>
 static hipError_t launch_lattice_probe_t(int q, int rev, void* a, void* b, int64_t n, int64_t ns, hipStream_t s) {
     const dim3 grid((unsigned)lbm_grid_blocks(n, 256));
+    const int tile = lbm_launch_tile((uint64_t)q * (uint64_t)ns * sizeof(E) * (b ? 2u : 1u));
     if (q == 19)
-        hipLaunchKernelGGL((k_lattice_probe<19, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns, rev);
+        hipLaunchKernelGGL((k_lattice_probe<19, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns, rev, tile);
     else
-        hipLaunchKernelGGL((k_lattice_probe<27, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns, rev);
+        hipLaunchKernelGGL((k_lattice_probe<27, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns, rev, tile);
     return hipGetLastError();
 }
 

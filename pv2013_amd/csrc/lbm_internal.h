@@ -65,8 +65,8 @@ struct StepArgs {
     const FaceTab* faces;
     const double* wvals;    // [slot][LBM_WCHUNK] waveform values of the current chunk
     int32_t wk;             // this launch's row in the chunk
-    int32_t rev;            // 1: logical blocks in descending order (within each XCD's range)
-    int32_t front;          // 1: logical blocks in dispatch order (no per-XCD ranges)
+    int32_t rev;            // 1: each XCD takes its logical blocks in descending order
+    int32_t tile;           // 0: one range of logical blocks per XCD; T: tiles of T blocks in turn
     const void* gin;        // [d][ns] post-collision populations
     void* gout;
     double omega;
@@ -136,8 +136,8 @@ hipError_t launch_stream_fill(float4* a, int64_t n, hipStream_t s);
 // lattice-shaped probe: q arrays of elem_bytes elements, stride ns; b == nullptr:
 // in place (reversed array order), else into lattice b; shift 1: reads (in place:
 // reads and writes) one node along, off their cache lines (needs ns >= n + 1);
-// rev 1: the blocks walk the lattice downward; 8-byte elements map blocks as the
-// FP64 step kernels do (one front, no per-XCD ranges)
+// rev 1: the blocks walk the lattice downward; blocks map to nodes as in the step
+// kernels (lbm_launch_tile of the bytes the pass touches)
 hipError_t launch_lattice_probe(int q, int elem_bytes, int shift, int rev, void* a, void* b, int64_t n, int64_t ns,
                                 hipStream_t s);
 hipError_t launch_wave_chunk(const WaveTab* waves, const int32_t* used, int nused, const int64_t* iter,
@@ -161,6 +161,11 @@ hipError_t launch_init_lp(int storage, int q, bool swap, const InitArgs& a, hipS
 
 // Grid rounded up to whole groups of 8 blocks so the XCD-aware block remap is a
 // bijection (blocks are dispatched round-robin over the 8 XCDs).
+// Tile of the block remap for a launch that touches `bytes` of lattice: one
+// range per XCD while the lattice fits the 256 MB Infinity Cache (neighbour
+// lines stay in one L2), else tiles of 16 blocks so the XCDs stream one front.
+constexpr uint64_t LBM_INFINITY_CACHE = (uint64_t)256 << 20;
+inline int lbm_launch_tile(uint64_t bytes) { return bytes > LBM_INFINITY_CACHE ? 16 : 0; }
 inline int lbm_grid_blocks(int64_t items, int block) {
     int64_t b = (items + block - 1) / block;
     return (int)(((b + 7) / 8) * 8);

@@ -37,17 +37,25 @@ struct PrecF16S {
     using calc_t = float;
 };
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of logical blocks so neighbouring rows share an L2.
-// front: logical block = dispatch order instead, one front through the
-// lattice (FP64, whose 64-lane accesses fill whole lines by themselves).
-// rev: the walk starts from the top, so a launch starts on the lines the
-// previous launch touched last.
-__device__ __forceinline__ int64_t logical_block(int rev = 0, int front = 0) {
-    if (front) return rev ? (int64_t)gridDim.x - 1 - blockIdx.x : (int64_t)blockIdx.x;
-    const int per = gridDim.x >> 3;
-    const int k = (int)(blockIdx.x >> 3);
-    return (int64_t)(blockIdx.x & 7) * per + (rev ? per - 1 - k : k);
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b & 7).
+// tile 0: each XCD takes one contiguous range of logical blocks, so
+// neighbouring rows share that XCD's L2.  tile T > 0: the XCDs take tiles of T
+// consecutive logical blocks in turn, so the eight of them advance together
+// as one front through the lattice (the blocks past the last whole round of
+// tiles split into one range per XCD).  The grid is a multiple of 8 blocks,
+// and either mapping is a bijection onto [0, gridDim.x).  rev: each XCD takes
+// its blocks from the top, so a launch starts on the lines the previous
+// launch touched last.
+__device__ __forceinline__ int64_t logical_block(int rev = 0, int tile = 0) {
+    const int64_t per = gridDim.x >> 3;  // blocks per XCD
+    const int x = (int)(blockIdx.x & 7);
+    int64_t k = blockIdx.x >> 3;         // this block's turn on its XCD
+    if (rev) k = per - 1 - k;
+    const int64_t T = tile > 0 && tile < per ? tile : per;
+    const int64_t full = per / T * T;
+    if (k < full) return ((k / T) * 8 + x) * T + k % T;
+    const int64_t r = per - full;
+    return 8 * full + x * r + (k - full);
 }
 
 // ------------------------------------------------------------- waveforms (FP64)
@@ -624,7 +632,7 @@ __global__ __launch_bounds__(512) void k_step(StepArgs a) {
     using L = Lat<Q>;
     using S = typename P::store_t;
     using T = typename P::calc_t;
-    const int64_t lb = logical_block(a.rev, a.front);
+    const int64_t lb = logical_block(a.rev, a.tile);
     const int64_t n = lb * blockDim.x + threadIdx.x;
     if (lb >= a.nblocks || n >= a.n) return;
     // gin and gout are the same buffer in the AA scheme, where a thread
@@ -954,7 +962,7 @@ template <int Q, class P, int MODE, int SCH, int PART, bool LAT = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     Q == 27 && sizeof(typename P::store_t) == 2 ? 5 : 1))) void k_step_rs(StepArgs a) {
     if constexpr (PART == PART_FUSED) {
-        const int64_t lb = logical_block(a.rev, a.front);
+        const int64_t lb = logical_block(a.rev, a.tile);
         if (lb >= a.nblocks) return;
         const int64_t j = __builtin_amdgcn_readfirstlane((int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6)));
         const int lane = (int)(threadIdx.x & 63);

@@ -227,8 +227,9 @@ typedef struct {
  * cut by a row end.  AA chunks are always one per wavefront.  Results are
  * identical. */
 #define LBM_OPT_ONE_CHUNK_PER_WAVE 256
-/* Every step launch walks the lattice upward.  By default the AA scheme's
- * gather step walks it downward, so each launch starts on the cache lines the
+/* Every step launch walks the lattice upward.  By default launches alternate
+ * direction (the AA gather walks downward, and the two-lattice pull when it
+ * reads the second lattice), so each launch starts on the cache lines the
  * previous launch touched last.  Results are identical. */
 #define LBM_OPT_FORWARD_ONLY 512
 

@@ -23,7 +23,7 @@ OPT_SPLIT_LAUNCH = 64  # records and row segments in two launches
 OPT_WIDE_RECORDS = 128  # FP16 chunks with records through 64-bit addresses (automatic >= 4 GiB)
 PROBE_TWO, PROBE_IN_PLACE, PROBE_TWO_SHIFTED, PROBE_IN_PLACE_SHIFTED, PROBE_AA = 0, 1, 2, 3, 4  # lbm_ctx_bandwidth_lattice
 OPT_ONE_CHUNK_PER_WAVE = 256  # two-lattice row chunks one per wavefront instead of two-chunk windows
-OPT_FORWARD_ONLY = 512  # every step launch walks the lattice upward (default: the AA gather walks it downward)
+OPT_FORWARD_ONLY = 512  # every step launch walks the lattice upward (default: launches alternate direction)
 BC_VELOCITY, BC_PRESSURE = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 WAVE_NONE, WAVE_FOURIER, WAVE_PV2013 = 0, 1, 2

@@ -392,10 +392,11 @@ static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* 
     a.wk = wk;
     a.gin = gin;
     a.gout = gout;
-    // AA: the gather step (sch 1) walks the lattice downward, so each launch
-    // starts on the lines the previous one touched last (the two-lattice pull
-    // keeps one direction: alternating it measured slower in FP32)
-    a.rev = D->scheme == LBM_SCHEME_AA && sch == 1 && !(D->options & LBM_OPT_FORWARD_ONLY);
+    // Launches alternate direction, so each starts on the lines the previous
+    // one touched last: the AA gather (sch 1) walks the lattice downward, and
+    // the two-lattice pull walks downward when it reads the second lattice.
+    a.rev = !(D->options & LBM_OPT_FORWARD_ONLY) &&
+            (D->scheme == LBM_SCHEME_AA ? sch == 1 : gin == D->g[1]);
     // lattices larger than the Infinity Cache stream as one front of 16-block
     // tiles (64-pipe ensemble FP32 1140 -> 1067 us per step), cache-resident
     // ones keep one range per XCD (benchmark case FP32 AA 66.0 against 68.5)

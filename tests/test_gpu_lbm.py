@@ -250,9 +250,9 @@ def test_aa_equals_two_lattice(P, oracle, ctx, storage):
     """The AA scheme (one lattice, alternating gather / local steps) computes
     exactly what the two-lattice pull computes, in every precision; with the
     graph path, the eager path, odd and even step counts.  The two-lattice pull
-    with one row chunk per wavefront equals its default (chunk pairs), and the
-    AA gather walking the lattice downward (the default; at 64 threads per
-    block the box spans several blocks per XCD) equals walking it upward."""
+    with one row chunk per wavefront equals its default (chunk pairs), and
+    launches alternating direction (the default; at 64 threads per block the
+    box spans several blocks per XCD) equal launches that all walk upward."""
     dims = (16, 16, 40)
     cl, bcs = pipe_case(oracle, dims, 6.0, 0.03, waveform=0)
     bc = [P.make_bc(**b) for b in bcs]
@@ -267,7 +267,9 @@ def test_aa_equals_two_lattice(P, oracle, ctx, storage):
     aa64 = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA, block=64)
     aa64_fwd = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, scheme=P.SCHEME_AA, block=64,
                         options=P.capi.OPT_FORWARD_ONLY)
-    others = (aa, aa_eager, aa_generic, aa_split, two_single, aa64, aa64_fwd)
+    two64 = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, block=64)
+    two64_fwd = P.Domain(ctx, dims, cl, 0.7, bc, storage=storage, block=64, options=P.capi.OPT_FORWARD_ONLY)
+    others = (aa, aa_eager, aa_generic, aa_split, two_single, aa64, aa64_fwd, two64, two64_fwd)
     for d in (two,) + others:
         d.set_waveform(0, P.WAVE_FOURIER, 200.0, a, phi)
     for n in (1, 70, 3, 130):

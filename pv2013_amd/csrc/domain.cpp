@@ -397,11 +397,12 @@ static int launch_step(lbm_domain* D, int mode, int sch, const void* gin, void* 
     // the two-lattice pull walks downward when it reads the second lattice.
     a.rev = !(D->options & LBM_OPT_FORWARD_ONLY) &&
             (D->scheme == LBM_SCHEME_AA ? sch == 1 : gin == D->g[1]);
-    // lattices larger than the Infinity Cache stream as one front of 16-block
+    // lattices larger than the Infinity Cache stream as one front of 64-wavefront
     // tiles (64-pipe ensemble FP32 1140 -> 1067 us per step), cache-resident
     // ones keep one range per XCD (benchmark case FP32 AA 66.0 against 68.5)
     a.tile = lbm_launch_tile((uint64_t)D->q * (uint64_t)D->ns * store_bytes(D->storage) *
-                             (D->scheme == LBM_SCHEME_TWO_LATTICE ? 2u : 1u));
+                                 (D->scheme == LBM_SCHEME_TWO_LATTICE ? 2u : 1u),
+                             D->block);
     a.fpre = D->fpre;
     a.rho = D->rho;
     a.u = D->u;

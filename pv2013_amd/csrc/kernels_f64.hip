@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_lattice_probe(E* a, E* b, int64_t n, in
 template <class E, int SHIFT>
 static hipError_t launch_lattice_probe_t(int q, int rev, void* a, void* b, int64_t n, int64_t ns, hipStream_t s) {
     const dim3 grid((unsigned)lbm_grid_blocks(n, 256));
-    const int tile = lbm_launch_tile((uint64_t)q * (uint64_t)ns * sizeof(E) * (b ? 2u : 1u));
+    const int tile = lbm_launch_tile((uint64_t)q * (uint64_t)ns * sizeof(E) * (b ? 2u : 1u), 256);
     if (q == 19)
         hipLaunchKernelGGL((k_lattice_probe<19, E, SHIFT>), grid, dim3(256), 0, s, (E*)a, (E*)b, n, ns, rev, tile);
     else

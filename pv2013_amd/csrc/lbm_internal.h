@@ -161,11 +161,14 @@ hipError_t launch_init_lp(int storage, int q, bool swap, const InitArgs& a, hipS
 
 // Grid rounded up to whole groups of 8 blocks so the XCD-aware block remap is a
 // bijection (blocks are dispatched round-robin over the 8 XCDs).
-// Tile of the block remap for a launch that touches `bytes` of lattice: one
-// range per XCD while the lattice fits the 256 MB Infinity Cache (neighbour
-// lines stay in one L2), else tiles of 16 blocks so the XCDs stream one front.
+// Tile of the block remap, in blocks, for a launch of `block`-thread blocks
+// that touches `bytes` of lattice: one range per XCD while the lattice fits the
+// 256 MB Infinity Cache (neighbour lines stay in one L2), else tiles of 64
+// wavefronts so the XCDs stream one front.
 constexpr uint64_t LBM_INFINITY_CACHE = (uint64_t)256 << 20;
-inline int lbm_launch_tile(uint64_t bytes) { return bytes > LBM_INFINITY_CACHE ? 16 : 0; }
+inline int lbm_launch_tile(uint64_t bytes, int block) {
+    return bytes > LBM_INFINITY_CACHE ? 64 * 64 / block : 0;
+}
 inline int lbm_grid_blocks(int64_t items, int block) {
     int64_t b = (items + block - 1) / block;
     return (int)(((b + 7) / 8) * 8);
